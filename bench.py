@@ -1,0 +1,245 @@
+"""Benchmark: NU points/s of 2D type-1 + type-2 NUFFT on one MI355X.
+
+Workload (the 2D radial configuration): M = 512 spokes x 7,812 samples = 3,999,744
+points, angles U[0, pi), radius U[-pi, pi), Gaussian jitter 1e-3, wrapped;
+complex64 strengths / modes N(0,1); 512 x 512 modes (1024 x 1024 fine grid),
+eps = 1e-6 (ns = 8), upsampfac 2.
+
+One step = nufft2d1 (bin-sort + spread + FFT + deconvolve) followed by
+nufft2d2 (bin-sort + deconvolve/pad + FFT + interp) on the same points, inputs
+resident in HBM.  value = 2 * M / step time (NU points per second over both
+transforms).  The roofline object is for the dominant kernel (spread or
+interp), timed with HIP events on the plan stream.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=250_000, help="points in the CPU-baseline sample")
+    ap.add_argument("--profile-only", action="store_true", help="run a few steps, print nothing but a status line")
+    return ap.parse_args()
+
+
+def direct_subset_type1(x, y, c, modes, isign, torch, dev):
+    """FP64 direct sum for a subset of (k1, k2) modes, on the GPU."""
+    xd = torch.as_tensor(x, dtype=torch.float64, device=dev)
+    yd = torch.as_tensor(y, dtype=torch.float64, device=dev)
+    cd = torch.as_tensor(c, dtype=torch.complex128, device=dev)
+    out = []
+    for k1, k2 in modes:
+        ph = isign * (k1 * xd + k2 * yd)
+        out.append(torch.sum(cd * torch.polar(torch.ones_like(ph), ph)))
+    return torch.stack(out).cpu().numpy()
+
+
+def direct_subset_type2(x, y, f, idx, isign, torch, dev):
+    n2, n1 = f.shape
+    k1 = torch.arange(-(n1 // 2), (n1 + 1) // 2, device=dev, dtype=torch.float64)
+    k2 = torch.arange(-(n2 // 2), (n2 + 1) // 2, device=dev, dtype=torch.float64)
+    fd = torch.as_tensor(f, dtype=torch.complex128, device=dev)
+    out = []
+    for j in idx:
+        ph = isign * (k2[:, None] * float(y[j]) + k1[None, :] * float(x[j]))
+        out.append(torch.sum(fd * torch.polar(torch.ones_like(ph), ph)))
+    return torch.stack(out).cpu().numpy()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_baseline(x, y, c, f, nm, eps, sample):
+    """The oracle (NumPy restatement of the reference's pure-JAX algorithm, complex64 like
+    the reference) on the first `sample` points of the same workload, 1 thread."""
+    from oracle import nufft_ref as R
+
+    xs, ys, cs = x[:sample], y[:sample], c[:sample]
+    R.nufft_type1([xs[:1000], ys[:1000]], cs[:1000], nm, eps=eps)  # warm
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        R.nufft_type1([xs, ys], cs, nm, eps=eps, isign=1)
+        R.nufft_type2([xs, ys], f, eps=eps, isign=-1)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {
+        "value": 2 * sample / t,
+        "unit": "NU pts/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"nufft2d1 + nufft2d2 on the first {sample} radial points (512x512 modes, complex64), "
+                  f"median of {len(times)} runs, {t:.2f} s each",
+    }
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU runs are not part of this "
+                                   "build (--gpus must be 1)"}))
+        sys.exit(2)
+    import torch
+
+    from nufftax_amd import workloads
+    from nufftax_amd._lib import lib, ptr, check
+    from nufftax_amd._plan import F32, Plan, transform_plan, grid_plan
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nm = (512, 512)
+    eps = 1e-6
+    x, y, c, f = workloads.radial_2d(seed=0, n_modes=nm)
+    M = x.size
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    c64 = c.astype(np.complex64)
+    f64 = f.astype(np.complex64)
+    xd = torch.from_numpy(x32).to(dev)
+    yd = torch.from_numpy(y32).to(dev)
+    cd = torch.from_numpy(c64).to(dev)
+    fd = torch.from_numpy(f64).to(dev)
+    fout = torch.empty((nm[1], nm[0]), dtype=torch.complex64, device=dev)
+    cout = torch.empty(M, dtype=torch.complex64, device=dev)
+
+    p1 = transform_plan(1, 2, nm, 1, eps, 1, 2.0, F32, dev)
+    p2 = transform_plan(2, 2, nm, 1, eps, -1, 2.0, F32, dev)
+    L = lib()
+
+    def setpts(p):
+        p._sync_stream()
+        check(L.nufftx_setpts(p.h, M, ptr(xd), ptr(yd), None, 0, None, None, None), "setpts")
+
+    def step():
+        setpts(p1)
+        check(L.nufftx_execute(p1.h, ptr(cd), ptr(fout)), "execute t1")
+        setpts(p2)
+        check(L.nufftx_execute(p2.h, ptr(cout), ptr(fd)), "execute t2")
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    if args.profile_only:
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        print("profile-only run done", flush=True)
+        return
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+
+    # per-transform and resident-points breakdown
+    def timeit(fn, n=10):
+        fn()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / n
+
+    t_t1 = timeit(lambda: (setpts(p1), check(L.nufftx_execute(p1.h, ptr(cd), ptr(fout)), "t1")))
+    t_t2 = timeit(lambda: (setpts(p2), check(L.nufftx_execute(p2.h, ptr(cout), ptr(fd)), "t2")))
+    setpts(p1)
+    setpts(p2)
+    t_t1_res = timeit(lambda: check(L.nufftx_execute(p1.h, ptr(cd), ptr(fout)), "t1"))
+    t_t2_res = timeit(lambda: check(L.nufftx_execute(p2.h, ptr(cout), ptr(fd)), "t2"))
+    t_sort = timeit(lambda: setpts(p1))
+
+    # dominant-kernel timing with HIP events on the plan stream (grid plan, same geometry)
+    from nufftax_amd._lib import kernel_params
+    ns, beta, kc = kernel_params(eps, 2.0)
+    nf = (p1.grid_size(0), p1.grid_size(1))
+    gp = grid_plan(2, nf, ns, beta, kc, 1, F32, dev)
+    gp.setpts([xd, yd])
+    grid = torch.zeros((nf[1], nf[0]), dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    reps = 20
+
+    def ev_time(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    t_spread = ev_time(lambda: gp.spread_add(cd, grid))
+    t_interp = ev_time(lambda: gp.interp(grid, cout))
+    nfc = nf[0] * nf[1]
+    bytes_spread = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c gathered; grid written once
+    bytes_interp = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c written; grid read once
+    if t_spread >= t_interp:
+        kname, kt, kb = "k_spread_tiled<float,float,2,8>", t_spread, bytes_spread
+    else:
+        kname, kt, kb = "k_gather_tiled<float,float,2,8,false>", t_interp, bytes_interp
+    achieved = kb / kt / 1e9
+
+    # accuracy against the FP64 direct NUDFT on subsets
+    rng = np.random.default_rng(1)
+    modes = [(int(a), int(b)) for a, b in zip(rng.integers(-256, 256, 64), rng.integers(-256, 256, 64))]
+    setpts(p1)
+    check(L.nufftx_execute(p1.h, ptr(cd), ptr(fout)), "t1")
+    fo = fout.cpu().numpy()
+    got1 = np.array([fo[k2 + 256, k1 + 256] for k1, k2 in modes])
+    ref1 = direct_subset_type1(x32, y32, c64, modes, 1, torch, dev)
+    idx = rng.integers(0, M, 64)
+    setpts(p2)
+    check(L.nufftx_execute(p2.h, ptr(cout), ptr(fd)), "t2")
+    got2 = cout.cpu().numpy()[idx]
+    ref2 = direct_subset_type2(x32, y32, f64, idx, -1, torch, dev)
+
+    res = {
+        "metric": "NU pts/s (2D type 1 + type 2, eps 1e-6)",
+        "value": 2 * M / t_step,
+        "unit": "NU pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "2D radial type-1 + type-2", "M": M, "n_modes": list(nm), "fine_grid": list(nf),
+                   "eps": eps, "nspread": ns, "upsampfac": 2.0},
+        "breakdown_ms": {"nufft2d1": t_t1 * 1e3, "nufft2d2": t_t2 * 1e3, "nufft2d1_resident_points": t_t1_res * 1e3,
+                         "nufft2d2_resident_points": t_t2_res * 1e3, "bin_sort": t_sort * 1e3,
+                         "spread_kernel": t_spread * 1e3, "interp_kernel": t_interp * 1e3},
+        "accuracy": {"nufft2d1_rel_l2_vs_direct_64_modes": rel_l2(got1, ref1),
+                     "nufft2d2_rel_l2_vs_direct_64_points": rel_l2(got2, ref2)},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": kb,
+                     "kernel_ms": kt * 1e3},
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(x32, y32, c64, f64, nm, eps, args.cpu_sample)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

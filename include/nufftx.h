@@ -1,0 +1,164 @@
+/*
+ * nufftx: MI355X (gfx950) implementation of nufftax's spread -> FFT -> deconvolve
+ * pipeline behind a plain C ABI.
+ *
+ * Every pointer argument named as data (coordinates, strengths, modes, grids,
+ * gradients) is a DEVICE pointer.  Complex data is interleaved (re, im) in the
+ * plan precision: complex64 for NUFFTX_F32, complex128 for NUFFTX_F64;
+ * coordinates are float32 / float64 respectively.  All work is enqueued on the
+ * plan's stream; no call synchronises the host except makeplan/setpts when they
+ * have to (re)allocate device buffers.  Every function returns a status code
+ * (NUFFTX_OK on success); nufftx_last_error() gives the message of the last
+ * failure on the calling thread.  Nothing aborts or exits.
+ *
+ * Layouts (nufftax conventions):
+ *   strengths / point values  c[n_trans][M]
+ *   modes  f[n_trans][n3][n2][n1]   (x fastest), CMCL order k in [-(n/2), (n-1)/2]
+ *          when modeord == 0, FFT order when modeord == 1
+ *   type-3 outputs            f[n_trans][N]
+ *
+ * Citations are to the reference Python package (GragasLab/nufftax).
+ */
+#ifndef NUFFTX_H
+#define NUFFTX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NUFFTX_VERSION 100
+
+enum nufftx_status {
+  NUFFTX_OK = 0,
+  NUFFTX_ERR_INVALID_ARG = 1,
+  NUFFTX_ERR_ALLOC = 2,
+  NUFFTX_ERR_HIP = 3,
+  NUFFTX_ERR_FFT = 4,
+  NUFFTX_ERR_STATE = 5
+};
+
+enum nufftx_precision { NUFFTX_F32 = 0, NUFFTX_F64 = 1 };
+
+typedef struct nufftx_plan_s* nufftx_plan;
+
+int nufftx_version(void);
+const char* nufftx_last_error(void);
+
+/* ---- host parameter helpers --------------------------------------------- */
+
+/* Replaces nufftax.core.kernel.compute_kernel_params (nufftax/core/kernel.py:165-206). */
+int nufftx_kernel_params(double eps, double upsampfac, int* nspread, double* beta, double* c);
+
+/* Replaces nufftax.utils.grid.compute_grid_size (nufftax/utils/grid.py:36-58); <0 on error. */
+int64_t nufftx_grid_size(int64_t n_modes, double upsampfac, int nspread);
+
+/* ---- transforms ----------------------------------------------------------- */
+
+/*
+ * Plan one transform family.  Replaces the per-call setup inside
+ * nufftax.transforms.nufft1.nufft{1,2,3}d1 (nufftax/transforms/nufft1.py:21-243),
+ * nufft2.nufft{1,2,3}d2 (nufft2.py:21-227) and nufft3.nufft{1,2,3}d3
+ * (nufft3.py:216-515): kernel parameters, fine-grid sizes, phihat / deconvolution
+ * factors, FFT plans.
+ *   type      1, 2 or 3
+ *   dim       1, 2 or 3
+ *   n_modes   type 1/2: mode counts (n1, n2, n3) -- x first;
+ *             type 3:   fine-grid sizes (nf1, nf2, nf3), the reference's n_modes
+ *   isign     sign of the exponent (+1 / -1)
+ *   n_trans   number of transforms sharing the points (leading batch axis), 1..65535
+ *   eps, upsampfac   as in the reference (static parameters)
+ *   precision NUFFTX_F32 or NUFFTX_F64
+ *   modeord   0 = CMCL (the public API), 1 = FFT order
+ *   stream    hipStream_t (NULL = default stream)
+ */
+int nufftx_makeplan(int type, int dim, const int64_t* n_modes, int isign, int64_t n_trans, double eps,
+                    double upsampfac, int precision, int modeord, void* stream, nufftx_plan* plan);
+
+/* Fine-grid size of dimension d (0-based) of a plan; for type 3 the inner type-2 grid if inner != 0. */
+int64_t nufftx_plan_grid_size(nufftx_plan plan, int d, int inner);
+
+int nufftx_set_stream(nufftx_plan plan, void* stream);
+
+/*
+ * Set the nonuniform points: folds coordinates in FP64, bin-sorts them and builds
+ * the subproblem list; the sort stays resident for every execute until the next
+ * setpts.  Coordinates are read again by execute, so they must stay valid and
+ * unchanged until then.  type 1/2: M points (x, y, z as dim requires);
+ * type 3: M sources (x, y, z) and N targets (s, t, u); the type-3 centring,
+ * rescaling and phase factors are computed here in FP64 (nufft3.py:257-299).
+ */
+int nufftx_setpts(nufftx_plan plan, int64_t M, const void* x, const void* y, const void* z, int64_t N,
+                  const void* s, const void* t, const void* u);
+
+/*
+ * Run the transform.  type 1: c -> f; type 2: f -> c; type 3: c -> f.
+ * Replaces the bodies of nufft{1,2,3}d1 (nufft1.py:51-89/126-162/201-243),
+ * nufft{1,2,3}d2 (nufft2.py:48-85/114-152/183-227) and nufft{1,2,3}d3 (nufft3.py).
+ */
+int nufftx_execute(nufftx_plan plan, void* c, void* f);
+
+/* Same as execute with the exponent sign overridden (type 1/2 only). */
+int nufftx_execute_sign(nufftx_plan plan, void* c, void* f, int isign);
+
+/*
+ * Reverse-mode derivative of a type-1 or type-2 plan (PyTorch cotangent
+ * convention: cotangent = dL/dRe + i dL/dIm).  Replaces the JVP + transpose
+ * rules of nufftax/transforms/primitives.py:88-110 (1D type 1), :127-151 (1D type 2),
+ * :170-194 / :212-241 (2D), :260-288 / :306-340 (3D).
+ *   type 1:  src = c[n_trans][M], g = cotangent of f  ->  grad_src = dL/dc (= type 2 with -isign of g),
+ *            grad_x/y/z[M] = sum_t Re(conj(c_t) * sum_grid G_t dW/dx), G = FFT_{-isign}(pad(g / phihat))
+ *   type 2:  src = f, g = cotangent of c[n_trans][M]  ->  grad_src = dL/df (= type 1 with -isign of g),
+ *            grad_x/y/z[M] = sum_t Re(conj(g_t) * sum_grid F_t dW/dx), F = FFT_{isign}(pad(f / phihat))
+ * Coordinate gradients use ES-kernel-derivative weights recomputed per point.
+ * Any output pointer may be NULL to skip it.  grad_x/y/z are overwritten.
+ */
+int nufftx_vjp(nufftx_plan plan, const void* src, const void* g, void* grad_src, void* grad_x, void* grad_y,
+               void* grad_z);
+
+int nufftx_destroy(nufftx_plan plan);
+
+/* ---- stage-level operators (nufftax.core) ------------------------------- */
+
+/*
+ * Plan spreading/interpolation on an explicit fine grid nf (x first) with an
+ * explicit ES kernel (nspread, beta, c): replaces the kernel_params argument of
+ * nufftax.core.spread_{1,2,3}d / interp_{1,2,3}d (nufftax/core/spread.py:766-1305).
+ * Points are set with nufftx_setpts (N ignored).
+ */
+int nufftx_makeplan_grid(int dim, const int64_t* nf, int nspread, double beta, double kc, int64_t n_trans,
+                         int precision, void* stream, nufftx_plan* plan);
+
+/* fw[n_trans][nf3][nf2][nf1] = spread(c)   (spread.py:285-339, 435-494, 596-658) */
+int nufftx_spread(nufftx_plan plan, const void* c, void* fw);
+
+/* fw += spread(c): accumulates into an existing grid (no clearing). */
+int nufftx_spread_add(nufftx_plan plan, const void* c, void* fw);
+
+/* c[n_trans][M] = interp(fw)                (spread.py:342-381, 497-540, 661-709) */
+int nufftx_interp(nufftx_plan plan, const void* fw, void* c);
+
+/*
+ * Gather with kernel-derivative weights (spread.py:820-944, 997-1114, 1168-1305):
+ *   gval[t][j] = sum G_t W_j                         (may be NULL)
+ *   grad_ax[j] = sum_t Re(conj(P_t[j]) * sum G_t dW_j/d ax)   (each may be NULL; overwritten)
+ * With P = c and G = grid cotangent this is the coordinate gradient of spread;
+ * with P = point cotangent and G = fw it is the coordinate gradient of interp.
+ */
+int nufftx_gather_grad(nufftx_plan plan, const void* G, const void* P, void* gval, void* grad_x, void* grad_y,
+                       void* grad_z);
+
+/* Bin geometry of a plan (inner != 0: the inner grid of a type-3 plan). */
+int nufftx_get_sort_info(nufftx_plan plan, int inner, int* nbins, int* bin_size /*[3]*/,
+                         int* n_bins_per_dim /*[3]*/);
+
+/* Copy the last setpts' sorted permutation (M or N int32) and bin starts
+   (nbins + 1 int32) into device buffers, on the plan stream.  Either may be NULL. */
+int nufftx_copy_sort(nufftx_plan plan, int inner, int32_t* perm, int32_t* bin_start);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUFFTX_H */

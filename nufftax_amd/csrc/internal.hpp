@@ -1,0 +1,119 @@
+// Internal interfaces between the nufftx translation units (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "common.hpp"
+
+namespace nufftx {
+
+enum Prec { kF32 = 0, kF64 = 1 };
+
+// Binning of one fine grid: bins of bs[d] cells, tiles of bs[d] + ns cells.
+struct Geometry {
+  int dim = 1;
+  int nf[3] = {1, 1, 1};
+  int bs[3] = {1, 1, 1};
+  int nb[3] = {1, 1, 1};
+  int nbins = 1;
+  int ns = 2;
+  int tile[3] = {1, 1, 1};  // bs + ns per used dim, 1 otherwise
+  int64_t tile_cells = 1;
+  int64_t grid_cells() const { return (int64_t)nf[0] * nf[1] * nf[2]; }
+};
+
+// A sorted set of points on one geometry.  Coordinates are not copied: the
+// kernels re-read them through perm and refold them in FP64.
+struct PointSet {
+  int64_t M = 0;
+  const void* coord[3] = {nullptr, nullptr, nullptr};
+  bool coord_f64 = false;
+  // sort state
+  int NB = 0;        // sort chunks (columns of the histogram matrix)
+  int CH = 0;        // points per chunk
+  int maxsub = 0;    // max points per subproblem
+  int max_subprob = 0;
+  // device buffers (owned)
+  int32_t* perm = nullptr;       // M
+  int32_t* hist = nullptr;       // nbins * NB (+ scan scratch)
+  int32_t* scan_part = nullptr;  // scan partials
+  int32_t* bin_start = nullptr;  // nbins + 1
+  int4* subprob = nullptr;       // max_subprob
+  int32_t* nsub = nullptr;       // 1
+  int64_t cap_M = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0;
+};
+
+struct SortTuning {
+  int chunk = 2048;            // points per histogram column
+  int64_t hist_cap = 8 << 20;  // max entries of the histogram matrix
+  int maxsub = 1024;           // points per subproblem
+};
+
+// Tile size / LDS budget helpers
+void choose_bins(Geometry& g, int prec, int lds_budget);
+int lds_budget_bytes();
+
+// ---- sort.hip
+hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err);
+void pointset_free(PointSet& ps);
+
+// ---- spreadinterp.hip
+struct GridOpArgs {
+  const PointSet* ps;
+  const Geometry* g;
+  int prec;
+  int64_t n_trans;
+  double beta, c;
+};
+// fw[n_trans][grid] += spread(c[n_trans][M]); fw must be zeroed by the caller.
+hipError_t launch_spread(const GridOpArgs& a, const void* c, void* fw, hipStream_t s);
+// c[n_trans][M] = interp(fw)
+hipError_t launch_interp(const GridOpArgs& a, const void* fw, void* c, hipStream_t s);
+// Fused gather with kernel-derivative weights.  For each point i and transform t:
+//   S0 = sum G w, S_ax = sum G dW/d ax   (dW/d ax includes the -nf/2pi chain factor)
+//   gval[t][i] = S0                               (if gval != nullptr)
+//   gcoord_ax[i] += Re(conj(P[t][i]) * S_ax)      (if gcoord[ax] != nullptr; zeroed by caller)
+hipError_t launch_interp_grad(const GridOpArgs& a, const void* G, const void* P, void* gval, void* const gcoord[3],
+                              hipStream_t s);
+
+// ---- elementwise.hip
+hipError_t launch_zero(void* p, int64_t bytes, hipStream_t s);
+// Type-1 mode extraction (deconvolve.py:81-366) and type-2 padding (deconvolve.py:114-468).
+struct DeconvArgs {
+  int dim;
+  int prec;
+  int64_t n_trans;
+  int nf[3];
+  int nm[3];
+  int modeord;
+  const void* fac[3];  // per dim, T, in output mode order
+};
+hipError_t launch_deconv_shuffle(const DeconvArgs& a, const void* fw_hat, void* f, hipStream_t s);
+hipError_t launch_deconv_pad(const DeconvArgs& a, const void* f, void* fw_hat, hipStream_t s);
+
+// ---- type 3 (elementwise.hip)
+struct T3State {  // lives in device memory, filled by launch_t3_setup
+  double C[3], D[3], gamma[3], h[3];
+};
+hipError_t launch_t3_setup(int dim, int prec, int64_t M, const void* const x[3], int64_t N, const void* const s[3],
+                           const int nf[3], double upsampfac, T3State* st, double* scratch, int scratch_elems,
+                           hipStream_t strm);
+// x'_d = (x_d - C_d)/gamma_d (double), pre[i] = exp(i isign sum_d D_d x_d) (complex double)
+hipError_t launch_t3_prep_src(int dim, int prec, int64_t M, const void* const x[3], const T3State* st, int isign,
+                              double* xr[3], double* pre, hipStream_t s);
+// s'_d = h_d gamma_d (s_d - D_d) (double), post[k] = exp(i isign sum (s-D) C) / prod phihat(s'_d)
+struct QuadRule {
+  int n;
+  double z[40];
+  double wphi[40];
+};
+hipError_t launch_t3_prep_tgt(int dim, int prec, int64_t N, const void* const s[3], const T3State* st, int isign,
+                              const QuadRule& q, double* sr[3], double* post, hipStream_t strm);
+// out[t][i] = in[t][i] * fac[i] (fac complex double), in/out complex T
+hipError_t launch_cmul_pointwise(int prec, int64_t n_trans, int64_t M, const void* in, const double* fac, void* out,
+                                 hipStream_t s);
+
+}  // namespace nufftx

@@ -1,0 +1,54 @@
+// Type-2 interpolation launchers (kernels in si_kernels.hpp).
+#include "si_kernels.hpp"
+
+namespace nufftx {
+
+namespace {
+
+template <typename T, typename CT, int D>
+hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
+  const SIArgs<T> a = make_si_args<T>(ga);
+  size_t lds = 0;
+  const bool tiled = use_tiles(*ga.g, ga.prec, lds);
+  const PointSet& ps = *ga.ps;
+  GatherOut<T, D, false> o;
+  o.val = static_cast<cplx<T>*>(c);
+  o.P = nullptr;
+  o.gcoord[0] = o.gcoord[1] = o.gcoord[2] = nullptr;
+  return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
+    constexpr int NS = decltype(nsc)::value;
+    if (tiled) {
+      auto kern = k_gather_tiled<T, CT, D, NS, false>;
+      if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
+                         static_cast<const cplx<T>*>(fw), o);
+    } else {
+      const unsigned nblk = (unsigned)((ps.M + 255) / 256);
+      hipLaunchKernelGGL((k_gather_global<T, CT, D, NS, false>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s,
+                         a, static_cast<const cplx<T>*>(fw), o);
+    }
+    return hipGetLastError();
+  });
+}
+
+template <typename T, typename CT>
+hipError_t interp_dim(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
+  switch (ga.g->dim) {
+    case 1: return interp_impl<T, CT, 1>(ga, fw, c, s);
+    case 2: return interp_impl<T, CT, 2>(ga, fw, c, s);
+    case 3: return interp_impl<T, CT, 3>(ga, fw, c, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_interp(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
+  if (ga.ps->M == 0) return hipSuccess;
+  if (ga.prec == kF32) {
+    return ga.ps->coord_f64 ? interp_dim<float, double>(ga, fw, c, s) : interp_dim<float, float>(ga, fw, c, s);
+  }
+  return interp_dim<double, double>(ga, fw, c, s);
+}
+
+}  // namespace nufftx

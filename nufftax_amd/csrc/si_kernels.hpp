@@ -1,0 +1,435 @@
+// Spreading (type 1) and interpolation (type 2) kernels on bin-sorted points.
+//
+// Math: fw[t, i] += c[t, j] * prod_d phi(i_d - xs_jd)   (nufftax/core/spread.py:435-494)
+//       c[t, j]   = sum_i fw[t, i] * prod_d phi(...)     (spread.py:497-540)
+// with i_d running over i0_d .. i0_d + ns - 1 (periodic), i0 = ceil(xs - ns/2).
+//
+// Tiled path: one workgroup per subproblem (<= maxsub points of one bin).  The
+// bin plus an ns-wide halo is a tile in LDS; spreading accumulates with LDS
+// atomics and flushes non-zero cells to HBM with global atomics, interpolation
+// stages the tile once and gathers from LDS.  Weights are evaluated per point
+// in registers (FP64 fold, T-precision ES kernel) and never stored.
+// Global path (tile larger than the LDS budget): same per-point work directly
+// on the fine grid in HBM, points visited in sorted order.
+#pragma once
+
+#include <type_traits>
+
+#include "internal.hpp"
+
+namespace nufftx {
+
+template <typename T>
+struct SIArgs {
+  const int32_t* perm;
+  const int4* subprob;
+  const int32_t* nsub;
+  const void* coord[3];
+  int64_t M;
+  int nf[3];
+  int bs[3];
+  int nb[3];
+  int tile[3];
+  int64_t tile_cells;
+  int64_t grid_cells;
+  EsArgs<T> es;
+  T dscale[3];  // -nf_d / (2 pi): chain factor of dz/dx
+};
+
+template <typename T, typename CT, int D, int NS>
+struct PointW {
+  T w[D][NS];
+  int i0[D];
+};
+
+template <typename T, typename CT, int D, int NS>
+__device__ __forceinline__ void point_weights(const SIArgs<T>& a, int p, T (&w)[D][NS], int (&i0)[D]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double x = (double)static_cast<const CT*>(a.coord[d])[p];
+    const Folded f = fold_coord(x, a.nf[d]);
+    i0[d] = support_start(f, NS / 2);
+    const T t = (T)((double)i0[d] - f.xs);
+    es_weights<T, NS>(t, a.es, w[d]);
+  }
+}
+
+template <typename T, typename CT, int D, int NS>
+__device__ __forceinline__ void point_weights_deriv(const SIArgs<T>& a, int p, T (&w)[D][NS], T (&dw)[D][NS],
+                                                    int (&i0)[D]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double x = (double)static_cast<const CT*>(a.coord[d])[p];
+    const Folded f = fold_coord(x, a.nf[d]);
+    i0[d] = support_start(f, NS / 2);
+    const T t = (T)((double)i0[d] - f.xs);
+    es_weights_deriv<T, NS>(t, a.es, w[d], dw[d]);
+  }
+}
+
+__device__ __forceinline__ int wrap_idx(int g, int n) {
+  g %= n;
+  return g < 0 ? g + n : g;
+}
+
+template <int D>
+__device__ __forceinline__ void tile_origin(const int4& sd, const int (&bs)[3], const int (&nb)[3], int half,
+                                            int (&org)[3]) {
+  int rem = sd.x;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int bc = rem % nb[d];
+    rem /= nb[d];
+    org[d] = d < D ? bc * bs[d] - half : 0;
+  }
+}
+
+template <int D>
+__device__ __forceinline__ int64_t tile_cell_to_global(int64_t cell, const int (&tile)[3], const int (&org)[3],
+                                                       const int (&nf)[3]) {
+  const int l0 = (int)(cell % tile[0]);
+  int64_t r = cell / tile[0];
+  const int l1 = (int)(r % tile[1]);
+  const int l2 = (int)(r / tile[1]);
+  const int g0 = wrap_idx(org[0] + l0, nf[0]);
+  if (D == 1) return g0;
+  const int g1 = wrap_idx(org[1] + l1, nf[1]);
+  if (D == 2) return (int64_t)g1 * nf[0] + g0;
+  const int g2 = wrap_idx(org[2] + l2, nf[2]);
+  return ((int64_t)g2 * nf[1] + g1) * nf[0] + g0;
+}
+
+// ------------------------------------------------------------------ spread (tiled)
+template <typename T, typename CT, int D, int NS>
+__global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>* __restrict__ c,
+                                                      cplx<T>* __restrict__ fw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sre = reinterpret_cast<T*>(smem_raw);
+  T* sim = sre + a.tile_cells;
+  const int sub = blockIdx.x;
+  if (sub >= *a.nsub) return;
+  const int tr = blockIdx.y;
+  const int4 sd = a.subprob[sub];
+  int org[3];
+  tile_origin<D>(sd, a.bs, a.nb, NS / 2, org);
+  for (int64_t i = threadIdx.x; i < a.tile_cells; i += blockDim.x) {
+    sre[i] = (T)0;
+    sim[i] = (T)0;
+  }
+  __syncthreads();
+  const cplx<T>* cT = c + (int64_t)tr * a.M;
+  const int t0 = a.tile[0], t1 = a.tile[1];
+  for (int j = threadIdx.x; j < sd.z; j += blockDim.x) {
+    const int p = a.perm[sd.y + j];
+    if ((unsigned)p >= (unsigned)a.M) continue;
+    const cplx<T> cv = cT[p];
+    T w[D][NS];
+    int i0[D];
+    point_weights<T, CT, D, NS>(a, p, w, i0);
+    const int l0 = i0[0] - org[0];
+    if constexpr (D == 1) {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        atomicAdd(sre + l0 + k, cv.re * w[0][k]);
+        atomicAdd(sim + l0 + k, cv.im * w[0][k]);
+      }
+    } else if constexpr (D == 2) {
+      const int l1 = i0[1] - org[1];
+#pragma unroll
+      for (int ky = 0; ky < NS; ++ky) {
+        const T vr = cv.re * w[1][ky], vi = cv.im * w[1][ky];
+        const int row = (l1 + ky) * t0 + l0;
+#pragma unroll
+        for (int kx = 0; kx < NS; ++kx) {
+          atomicAdd(sre + row + kx, vr * w[0][kx]);
+          atomicAdd(sim + row + kx, vi * w[0][kx]);
+        }
+      }
+    } else {
+      const int l1 = i0[1] - org[1];
+      const int l2 = i0[2] - org[2];
+#pragma unroll 1
+      for (int kz = 0; kz < NS; ++kz) {
+        const T zr = cv.re * w[2][kz], zi = cv.im * w[2][kz];
+#pragma unroll
+        for (int ky = 0; ky < NS; ++ky) {
+          const T vr = zr * w[1][ky], vi = zi * w[1][ky];
+          const int row = ((l2 + kz) * t1 + (l1 + ky)) * t0 + l0;
+#pragma unroll
+          for (int kx = 0; kx < NS; ++kx) {
+            atomicAdd(sre + row + kx, vr * w[0][kx]);
+            atomicAdd(sim + row + kx, vi * w[0][kx]);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  cplx<T>* fT = fw + (int64_t)tr * a.grid_cells;
+  for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x) {
+    const T vr = sre[cell], vi = sim[cell];
+    if (vr == (T)0 && vi == (T)0) continue;
+    const int64_t gi = tile_cell_to_global<D>(cell, a.tile, org, a.nf);
+    atomic_add_global(&fT[gi].re, vr);
+    atomic_add_global(&fT[gi].im, vi);
+  }
+}
+
+// ------------------------------------------------------------------ spread (global)
+template <typename T, typename CT, int D, int NS>
+__global__ void __launch_bounds__(256) k_spread_global(SIArgs<T> a, const cplx<T>* __restrict__ c,
+                                                       cplx<T>* __restrict__ fw) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.M) return;
+  const int tr = blockIdx.y;
+  const int p = a.perm[j];
+  if ((unsigned)p >= (unsigned)a.M) return;
+  const cplx<T> cv = c[(int64_t)tr * a.M + p];
+  T w[D][NS];
+  int i0[D];
+  point_weights<T, CT, D, NS>(a, p, w, i0);
+  cplx<T>* fT = fw + (int64_t)tr * a.grid_cells;
+  int gx[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) gx[k] = wrap_idx(i0[0] + k, a.nf[0]);
+  if constexpr (D == 1) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      atomic_add_global(&fT[gx[k]].re, cv.re * w[0][k]);
+      atomic_add_global(&fT[gx[k]].im, cv.im * w[0][k]);
+    }
+  } else {
+    const int kzn = D == 3 ? NS : 1;
+#pragma unroll 1
+    for (int kz = 0; kz < kzn; ++kz) {
+      T zr = cv.re, zi = cv.im;
+      int64_t zoff = 0;
+      if constexpr (D == 3) {
+        zr *= w[2][kz];
+        zi *= w[2][kz];
+        zoff = (int64_t)wrap_idx(i0[2] + kz, a.nf[2]) * a.nf[1];
+      }
+#pragma unroll
+      for (int ky = 0; ky < NS; ++ky) {
+        const T vr = zr * w[1][ky], vi = zi * w[1][ky];
+        const int64_t row = (zoff + wrap_idx(i0[1] + ky, a.nf[1])) * a.nf[0];
+#pragma unroll
+        for (int kx = 0; kx < NS; ++kx) {
+          atomic_add_global(&fT[row + gx[kx]].re, vr * w[0][kx]);
+          atomic_add_global(&fT[row + gx[kx]].im, vi * w[0][kx]);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ gather helpers
+// Accumulates sum over the support of G[cell] * wx[kx] * wy[ky] * wz[kz] for
+// value weights, and optionally the per-dimension derivative sums.
+template <typename T, int D, int NS, bool DERIV, typename Fetch>
+__device__ __forceinline__ void gather_point(const T (&w)[D][NS], const T (&dw)[D][NS], Fetch fetch, cplx<T>& s0,
+                                             cplx<T> (&sd)[D]) {
+  s0 = {0, 0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) sd[d] = {0, 0};
+  if constexpr (D == 1) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const cplx<T> v = fetch(0, 0, k);
+      s0.re += v.re * w[0][k];
+      s0.im += v.im * w[0][k];
+      if constexpr (DERIV) {
+        sd[0].re += v.re * dw[0][k];
+        sd[0].im += v.im * dw[0][k];
+      }
+    }
+  } else {
+    const int kzn = D == 3 ? NS : 1;
+#pragma unroll 1
+    for (int kz = 0; kz < kzn; ++kz) {
+      cplx<T> p0 = {0, 0}, px = {0, 0}, py = {0, 0};
+#pragma unroll
+      for (int ky = 0; ky < NS; ++ky) {
+        cplx<T> r0 = {0, 0}, rx = {0, 0};
+#pragma unroll
+        for (int kx = 0; kx < NS; ++kx) {
+          const cplx<T> v = fetch(kz, ky, kx);
+          r0.re += v.re * w[0][kx];
+          r0.im += v.im * w[0][kx];
+          if constexpr (DERIV) {
+            rx.re += v.re * dw[0][kx];
+            rx.im += v.im * dw[0][kx];
+          }
+        }
+        p0.re += r0.re * w[1][ky];
+        p0.im += r0.im * w[1][ky];
+        if constexpr (DERIV) {
+          px.re += rx.re * w[1][ky];
+          px.im += rx.im * w[1][ky];
+          py.re += r0.re * dw[1][ky];
+          py.im += r0.im * dw[1][ky];
+        }
+      }
+      if constexpr (D == 2) {
+        s0 = p0;
+        if constexpr (DERIV) {
+          sd[0] = px;
+          sd[1] = py;
+        }
+      } else {
+        const T wz = w[D - 1][kz];
+        s0.re += p0.re * wz;
+        s0.im += p0.im * wz;
+        if constexpr (DERIV) {
+          const T dwz = dw[D - 1][kz];
+          sd[0].re += px.re * wz;
+          sd[0].im += px.im * wz;
+          sd[1].re += py.re * wz;
+          sd[1].im += py.im * wz;
+          sd[D - 1].re += p0.re * dwz;
+          sd[D - 1].im += p0.im * dwz;
+        }
+      }
+    }
+  }
+}
+
+// Output stage shared by interp and the gradient gather.
+template <typename T, int D, bool DERIV>
+struct GatherOut {
+  cplx<T>* val;           // [n_trans][M] or nullptr
+  const cplx<T>* P;       // [n_trans][M] pointwise factor for coordinate gradients
+  T* gcoord[3];           // [M] each, accumulated, or nullptr
+};
+
+template <typename T, int D, bool DERIV>
+__device__ __forceinline__ void emit_point(const SIArgs<T>& a, const GatherOut<T, D, DERIV>& o, int tr, int p,
+                                           const cplx<T>& s0, const cplx<T> (&sd)[D]) {
+  const int64_t ip = (int64_t)tr * a.M + p;
+  if (o.val) o.val[ip] = s0;
+  if constexpr (DERIV) {
+    const cplx<T> pv = o.P[ip];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (o.gcoord[d]) {
+        // Re(conj(P) * S) with the dz/dx chain factor
+        const T g = (pv.re * sd[d].re + pv.im * sd[d].im) * a.dscale[d];
+        atomic_add_global(o.gcoord[d] + p, g);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ interp / grad (tiled)
+template <typename T, typename CT, int D, int NS, bool DERIV>
+__global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>* __restrict__ fw,
+                                                      GatherOut<T, D, DERIV> o) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int sub = blockIdx.x;
+  if (sub >= *a.nsub) return;
+  const int tr = blockIdx.y;
+  const int4 sd4 = a.subprob[sub];
+  int org[3];
+  tile_origin<D>(sd4, a.bs, a.nb, NS / 2, org);
+  const cplx<T>* fT = fw + (int64_t)tr * a.grid_cells;
+  for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x)
+    tile[cell] = fT[tile_cell_to_global<D>(cell, a.tile, org, a.nf)];
+  __syncthreads();
+  const int t0 = a.tile[0], t1 = a.tile[1];
+  for (int j = threadIdx.x; j < sd4.z; j += blockDim.x) {
+    const int p = a.perm[sd4.y + j];
+    if ((unsigned)p >= (unsigned)a.M) continue;
+    T w[D][NS], dw[D][NS];
+    int i0[D];
+    if constexpr (DERIV)
+      point_weights_deriv<T, CT, D, NS>(a, p, w, dw, i0);
+    else
+      point_weights<T, CT, D, NS>(a, p, w, i0);
+    int l[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) l[d] = i0[d] - org[d];
+    auto fetch = [&](int kz, int ky, int kx) -> cplx<T> {
+      return tile[((int64_t)(l[2] + kz) * t1 + (l[1] + ky)) * t0 + l[0] + kx];
+    };
+    cplx<T> s0, sdd[D];
+    gather_point<T, D, NS, DERIV>(w, dw, fetch, s0, sdd);
+    emit_point<T, D, DERIV>(a, o, tr, p, s0, sdd);
+  }
+}
+
+template <typename T, typename CT, int D, int NS, bool DERIV>
+__global__ void __launch_bounds__(256) k_gather_global(SIArgs<T> a, const cplx<T>* __restrict__ fw,
+                                                       GatherOut<T, D, DERIV> o) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.M) return;
+  const int tr = blockIdx.y;
+  const int p = a.perm[j];
+  if ((unsigned)p >= (unsigned)a.M) return;
+  T w[D][NS], dw[D][NS];
+  int i0[D];
+  if constexpr (DERIV)
+    point_weights_deriv<T, CT, D, NS>(a, p, w, dw, i0);
+  else
+    point_weights<T, CT, D, NS>(a, p, w, i0);
+  const cplx<T>* fT = fw + (int64_t)tr * a.grid_cells;
+  auto fetch = [&](int kz, int ky, int kx) -> cplx<T> {
+    int64_t idx = wrap_idx(i0[0] + kx, a.nf[0]);
+    if (D >= 2) idx += (int64_t)wrap_idx(i0[D >= 2 ? 1 : 0] + ky, a.nf[1]) * a.nf[0];
+    if (D == 3) idx += (int64_t)wrap_idx(i0[D - 1] + kz, a.nf[2]) * a.nf[1] * a.nf[0];
+    return fT[idx];
+  };
+  cplx<T> s0, sdd[D];
+  gather_point<T, D, NS, DERIV>(w, dw, fetch, s0, sdd);
+  emit_point<T, D, DERIV>(a, o, tr, p, s0, sdd);
+}
+
+// ------------------------------------------------------------------ host-side helpers
+template <typename F>
+hipError_t dispatch_ns(int ns, F&& f) {
+  switch (ns) {
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 6: return f(std::integral_constant<int, 6>{});
+    case 8: return f(std::integral_constant<int, 8>{});
+    case 10: return f(std::integral_constant<int, 10>{});
+    case 12: return f(std::integral_constant<int, 12>{});
+    case 14: return f(std::integral_constant<int, 14>{});
+    case 16: return f(std::integral_constant<int, 16>{});
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+SIArgs<T> make_si_args(const GridOpArgs& ga) {
+  const PointSet& ps = *ga.ps;
+  const Geometry& g = *ga.g;
+  SIArgs<T> a;
+  a.perm = ps.perm;
+  a.subprob = ps.subprob;
+  a.nsub = ps.nsub;
+  for (int d = 0; d < 3; ++d) {
+    a.coord[d] = ps.coord[d];
+    a.nf[d] = g.nf[d];
+    a.bs[d] = g.bs[d];
+    a.nb[d] = g.nb[d];
+    a.tile[d] = g.tile[d];
+    a.dscale[d] = (T)(-(double)g.nf[d] / kTwoPi);
+  }
+  a.M = ps.M;
+  a.tile_cells = g.tile_cells;
+  a.grid_cells = g.grid_cells();
+  const double log2e = 1.4426950408889634074;
+  a.es.beta_log2e = std::is_same<T, float>::value ? (T)(ga.beta * log2e) : (T)ga.beta;
+  a.es.beta = (T)ga.beta;
+  a.es.c = (T)ga.c;
+  return a;
+}
+
+inline bool use_tiles(const Geometry& g, int prec, size_t& lds) {
+  const size_t cell = prec == kF32 ? 8 : 16;
+  lds = (size_t)g.tile_cells * cell;
+  return lds <= (size_t)lds_budget_bytes();
+}
+
+}  // namespace nufftx

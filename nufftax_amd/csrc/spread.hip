@@ -1,0 +1,50 @@
+// Type-1 spreading launchers (kernels in si_kernels.hpp).
+#include "si_kernels.hpp"
+
+namespace nufftx {
+
+namespace {
+
+template <typename T, typename CT, int D>
+hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
+  const SIArgs<T> a = make_si_args<T>(ga);
+  size_t lds = 0;
+  const bool tiled = use_tiles(*ga.g, ga.prec, lds);
+  const PointSet& ps = *ga.ps;
+  return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
+    constexpr int NS = decltype(nsc)::value;
+    if (tiled) {
+      auto kern = k_spread_tiled<T, CT, D, NS>;
+      if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
+                         static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
+    } else {
+      const unsigned nblk = (unsigned)((ps.M + 255) / 256);
+      hipLaunchKernelGGL((k_spread_global<T, CT, D, NS>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s, a,
+                         static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
+    }
+    return hipGetLastError();
+  });
+}
+
+template <typename T, typename CT>
+hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
+  switch (ga.g->dim) {
+    case 1: return spread_impl<T, CT, 1>(ga, c, fw, s);
+    case 2: return spread_impl<T, CT, 2>(ga, c, fw, s);
+    case 3: return spread_impl<T, CT, 3>(ga, c, fw, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_spread(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
+  if (ga.ps->M == 0) return hipSuccess;
+  if (ga.prec == kF32) {
+    return ga.ps->coord_f64 ? spread_dim<float, double>(ga, c, fw, s) : spread_dim<float, float>(ga, c, fw, s);
+  }
+  return spread_dim<double, double>(ga, c, fw, s);
+}
+
+}  // namespace nufftx

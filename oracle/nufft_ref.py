@@ -593,7 +593,7 @@ def bin_keys(coords, nfs, bin_sizes) -> np.ndarray:
         xs = r * nfs[ax]
         xs = np.where(xs >= nfs[ax], xs - nfs[ax], xs)
         nb = -(-nfs[ax] // bin_sizes[ax])
-        b = np.floor(xs / bin_sizes[ax]).astype(np.int64)
+        b = np.floor(xs).astype(np.int64) // bin_sizes[ax]
         key += b * stride
         stride *= nb
     return key

@@ -1,0 +1,307 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle.
+
+Tolerances:
+  * bin-sort: bit-exact (perm == argsort(keys, stable), bin_start == cumsum(bincount));
+  * spread/interp vs the FP64 oracle: rel l2 < 2e-6 in FP32, < 1e-12 in FP64;
+  * transforms: relative l2 error against the FP64 direct NUDFT no worse than
+    2x the error of the complex64 restatement of the reference algorithm (the
+    reference-error proxy) plus 1e-7, and < 1e-10 * 10 in FP64 at eps 1e-12.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import nufft_ref as R
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _t(a, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(DEV)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import nufftax_amd  # noqa: F401
+
+
+# ------------------------------------------------------------------ bin sort
+
+
+@pytest.mark.parametrize("dim,n_modes,M", [(1, (1000,), 50000), (2, (100, 70), 30000), (2, (512, 512), 200000),
+                                           (3, (40, 32, 24), 40000)])
+@pytest.mark.parametrize("dist", ["uniform", "clustered"])
+def test_bin_sort_bit_exact(rng, dim, n_modes, M, dist):
+    from nufftax_amd._plan import F32, transform_plan
+
+    if dist == "uniform":
+        coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(dim)]
+    else:  # a dense spot plus a radial line: many equal keys in a row
+        coords = [np.concatenate([rng.normal(0.0, 0.01, M // 2), np.linspace(-np.pi, np.pi, M - M // 2)])
+                  for _ in range(dim)]
+    coords[0][:5] = [-np.pi, np.pi, 0.0, -1e-300, 3 * np.pi]  # edges and wrap
+    xs = [_t(c.astype(np.float32)) for c in coords]
+    plan = transform_plan(1, dim, n_modes, 1, 1e-6, 1, 2.0, F32, torch.device(DEV))
+    plan.setpts(xs)
+    perm, bst, nbins, bs, nbd = plan.sort_state()
+    torch.cuda.synchronize()
+    nfs = [plan.grid_size(d) for d in range(dim)]
+    keys = R.bin_keys([c.astype(np.float32).astype(np.float64) for c in coords], nfs, bs[:dim])
+    assert int(np.prod(nbd[:dim])) == nbins
+    ref_perm, ref_off = R.bin_sort(keys, nbins)
+    np.testing.assert_array_equal(_np(perm).astype(np.int64), ref_perm)
+    np.testing.assert_array_equal(_np(bst).astype(np.int64), ref_off)
+
+
+# ------------------------------------------------------------------ spread / interp
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("dim,nfs", [(1, (300,)), (2, (96, 64)), (3, (40, 36, 32))])
+def test_spread_interp_vs_oracle(rng, prec, dim, nfs):
+    from nufftax_amd._plan import F32, F64, grid_plan
+
+    rdt = np.float32 if prec == "f32" else np.float64
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    kp = R.compute_kernel_params(1e-6 if prec == "f32" else 1e-12)
+    M = 5000
+    coords = [rng.uniform(-np.pi, np.pi, M).astype(rdt) for _ in range(dim)]
+    c = (rng.standard_normal((2, M)) + 1j * rng.standard_normal((2, M))).astype(cdt)
+    plan = grid_plan(dim, nfs, kp.nspread, kp.beta, kp.c, 2, F32 if prec == "f32" else F64, torch.device(DEV))
+    xs = [_t(x) for x in coords]
+    plan.setpts(xs)
+    fw = torch.empty((2,) + tuple(reversed(nfs)), dtype=torch.complex64 if prec == "f32" else torch.complex128,
+                     device=DEV)
+    plan.spread(_t(c), fw)
+    ref = R.spread([x.astype(np.float64) for x in coords], c.astype(np.complex128), nfs, kp)
+    tol = 2e-6 if prec == "f32" else 1e-12
+    assert R.rel_l2(_np(fw), ref) < tol
+    g = (rng.standard_normal((2,) + tuple(reversed(nfs))) + 1j * rng.standard_normal((2,) + tuple(reversed(nfs))))
+    out = torch.empty((2, M), dtype=fw.dtype, device=DEV)
+    plan.interp(_t(g.astype(cdt)), out)
+    refi = R.interp([x.astype(np.float64) for x in coords], g.astype(cdt).astype(np.complex128), kp)
+    assert R.rel_l2(_np(out), refi) < tol
+
+
+def test_spread_global_path_matches_tiled(rng, monkeypatch):
+    """A tile larger than the LDS budget takes the global-memory path: same result."""
+    from nufftax_amd._plan import F64, grid_plan
+
+    kp = R.compute_kernel_params(1e-14)  # ns = 16 in 3D FP64: tile does not fit LDS
+    nfs = (40, 36, 34)
+    M = 3000
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(3)]
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    plan = grid_plan(3, nfs, kp.nspread, kp.beta, kp.c, 1, F64, torch.device(DEV))
+    plan.setpts([_t(x) for x in coords])
+    fw = torch.empty(tuple(reversed(nfs)), dtype=torch.complex128, device=DEV)
+    plan.spread(_t(c), fw)
+    ref = R.spread(coords, c, nfs, kp)
+    assert R.rel_l2(_np(fw), ref) < 1e-12
+
+
+# ------------------------------------------------------------------ transforms vs direct NUDFT
+
+
+def _proxy_err_type1(coords, c, nm, isign, eps):
+    """Error of the complex64 restatement of the reference algorithm vs the FP64 NUDFT."""
+    f32 = R.nufft_type1([x.astype(np.float32) for x in coords], c.astype(np.complex64), nm, eps=eps, isign=isign)
+    return R.rel_l2(f32, R.direct_type1(coords, c, nm, isign=isign))
+
+
+@pytest.mark.parametrize("dim,nm", [(1, (64,)), (1, (33,)), (2, (32, 24)), (2, (17, 20)), (3, (12, 10, 16))])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_type1_vs_direct_c64(rng, dim, nm, isign):
+    import nufftax_amd as N
+
+    M = 2000
+    coords = [rng.uniform(-np.pi, np.pi, M).astype(np.float32).astype(np.float64) for _ in range(dim)]
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64).astype(np.complex128)
+    fn = [N.nufft1d1, N.nufft2d1, N.nufft3d1][dim - 1]
+    args = [_t(x, torch.float32) for x in coords] + [_t(c, torch.complex64)]
+    out = fn(*args, nm if dim > 1 else nm[0], eps=1e-6, isign=isign)
+    assert out.dtype == torch.complex64 and tuple(out.shape) == tuple(reversed(nm))
+    truth = R.direct_type1(coords, c, nm, isign=isign)
+    err = R.rel_l2(_np(out), truth)
+    proxy = _proxy_err_type1(coords, c, nm, isign, 1e-6)
+    assert err <= 2 * proxy + 1e-7, (err, proxy)
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("dim,nm", [(1, (40,)), (2, (24, 18)), (3, (10, 12, 8))])
+@pytest.mark.parametrize("isign", [1, -1])
+def test_type2_vs_direct_c64(rng, dim, nm, isign):
+    import nufftax_amd as N
+
+    M = 1500
+    coords = [rng.uniform(-np.pi, np.pi, M).astype(np.float32).astype(np.float64) for _ in range(dim)]
+    f = (rng.standard_normal(tuple(reversed(nm))) + 1j * rng.standard_normal(tuple(reversed(nm))))
+    f = f.astype(np.complex64).astype(np.complex128)
+    fn = [N.nufft1d2, N.nufft2d2, N.nufft3d2][dim - 1]
+    out = fn(*[_t(x, torch.float32) for x in coords], _t(f, torch.complex64), eps=1e-6, isign=isign)
+    assert out.dtype == torch.complex64 and tuple(out.shape) == (M,)
+    truth = R.direct_type2(coords, f, isign=isign)
+    err = R.rel_l2(_np(out), truth)
+    proxy = R.rel_l2(R.nufft_type2([x.astype(np.float32) for x in coords], f.astype(np.complex64), eps=1e-6,
+                                   isign=isign), truth)
+    assert err <= 2 * proxy + 1e-7, (err, proxy)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_type12_c128_high_precision(rng, dim):
+    import nufftax_amd as N
+
+    nm = [(50,), (20, 16), (10, 8, 12)][dim - 1]
+    M = 800
+    coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(dim)]
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    f1 = [N.nufft1d1, N.nufft2d1, N.nufft3d1][dim - 1]
+    out = f1(*[_t(x) for x in coords], _t(c), nm if dim > 1 else nm[0], eps=1e-12)
+    assert out.dtype == torch.complex128
+    assert R.rel_l2(_np(out), R.direct_type1(coords, c, nm)) < 1e-10
+    f = rng.standard_normal(tuple(reversed(nm))) + 1j * rng.standard_normal(tuple(reversed(nm)))
+    f2 = [N.nufft1d2, N.nufft2d2, N.nufft3d2][dim - 1]
+    out2 = f2(*[_t(x) for x in coords], _t(f), eps=1e-12)
+    assert R.rel_l2(_np(out2), R.direct_type2(coords, f)) < 1e-10
+
+
+def test_complex64_small_eps_runs_in_fp64(rng):
+    import nufftax_amd as N
+
+    M, n = 500, 40
+    x = rng.uniform(-np.pi, np.pi, M).astype(np.float32).astype(np.float64)
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64).astype(np.complex128)
+    out = N.nufft1d1(_t(x, torch.float32), _t(c, torch.complex64), n, eps=1e-9)
+    assert out.dtype == torch.complex64
+    # FP64 pipeline, rounded to complex64 at the end
+    assert R.rel_l2(_np(out), R.direct_type1([x], c, (n,))) < 3e-7
+
+
+def test_matches_oracle_restatement_fp64(rng):
+    """Same algorithm in FP64: GPU == oracle to rounding (pins deconvolution/FFT mapping)."""
+    import nufftax_amd as N
+
+    M = 3000
+    x, y = (rng.uniform(-4, 4, M) for _ in range(2))
+    c = rng.standard_normal((3, M)) + 1j * rng.standard_normal((3, M))
+    out = N.nufft2d1(_t(x), _t(y), _t(c), (30, 22), eps=1e-6, isign=-1, upsampfac=1.25)
+    ref = R.nufft_type1([x, y], c, (30, 22), eps=1e-6, isign=-1, upsampfac=1.25)
+    assert out.shape == (3, 22, 30)
+    assert R.rel_l2(_np(out), ref) < 1e-12
+    f = rng.standard_normal((2, 22, 30)) + 1j * rng.standard_normal((2, 22, 30))
+    out2 = N.nufft2d2(_t(x), _t(y), _t(f), eps=1e-6, isign=1, upsampfac=1.25)
+    ref2 = R.nufft_type2([x, y], f, eps=1e-6, isign=1, upsampfac=1.25)
+    assert R.rel_l2(_np(out2), ref2) < 1e-12
+
+
+# ------------------------------------------------------------------ closed forms and edge cases
+
+
+def test_closed_forms():
+    import nufftax_amd as N
+
+    k = np.arange(-8, 8)
+    out = N.nufft1d1(_t(np.array([0.5])), _t(np.array([1.0 + 1j])), 16, eps=1e-6)
+    np.testing.assert_allclose(_np(out), (1 + 1j) * np.exp(1j * k * 0.5), rtol=1e-5)
+    out = N.nufft1d1(_t(np.full(10, np.pi, np.float32)), _t(np.ones(10, np.complex64)), 32, eps=1e-6)
+    np.testing.assert_allclose(_np(out), 10 * np.exp(1j * np.arange(-16, 16) * np.pi), rtol=1e-5, atol=1e-5)
+    x = np.linspace(-np.pi, np.pi, 32, endpoint=False)
+    out = _np(N.nufft1d1(_t(x), _t(np.ones(32, complex)), 32, eps=1e-6))
+    assert abs(out[16]) == pytest.approx(32, rel=1e-5) and np.max(np.abs(np.delete(out, 16))) < 1e-4
+    out2 = N.nufft2d1(_t(np.array([0.3])), _t(np.array([-1.1])), _t(np.array([2.0 + 0j])), (6, 4))
+    k1, k2 = np.arange(-3, 3), np.arange(-2, 2)
+    np.testing.assert_allclose(_np(out2), 2 * np.exp(1j * (k2[:, None] * -1.1 + k1[None, :] * 0.3)), rtol=1e-5)
+
+
+def test_wrap_invariance_and_empty(rng):
+    import nufftax_amd as N
+
+    x = rng.uniform(np.pi, 3 * np.pi, 200)
+    c = rng.standard_normal(200) + 1j * rng.standard_normal(200)
+    a = N.nufft1d1(_t(x), _t(c), 32)
+    b = N.nufft1d1(_t(R.wrap_pi(x)), _t(c), 32)
+    np.testing.assert_allclose(_np(a), _np(b), rtol=1e-9, atol=1e-9)
+    z = N.nufft2d1(_t(np.zeros(0)), _t(np.zeros(0)), _t(np.zeros(0, complex)), (8, 6))
+    assert z.shape == (6, 8) and float(z.abs().max()) == 0.0
+    e = N.nufft2d2(_t(np.zeros(0)), _t(np.zeros(0)), _t(np.ones((6, 8), complex)))
+    assert e.shape == (0,)
+
+
+def test_batched_n_trans_matches_loop(rng):
+    import nufftax_amd as N
+
+    M = 4000
+    x, y = (_t(rng.uniform(-np.pi, np.pi, M).astype(np.float32)) for _ in range(2))
+    c = _t((rng.standard_normal((2, 3, M)) + 1j * rng.standard_normal((2, 3, M))).astype(np.complex64))
+    out = N.nufft2d1(x, y, c, (40, 30))
+    assert out.shape == (2, 3, 30, 40)
+    for i in range(2):
+        for j in range(3):
+            single = N.nufft2d1(x, y, c[i, j], (40, 30))
+            assert R.rel_l2(_np(out[i, j]), _np(single)) < 1e-6
+
+
+def test_type1_type2_adjoint(rng):
+    import nufftax_amd as N
+
+    M = 3000
+    x, y, z = (rng.uniform(-np.pi, np.pi, M) for _ in range(3))
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    f = rng.standard_normal((6, 8, 10)) + 1j * rng.standard_normal((6, 8, 10))
+    lhs = np.vdot(f, _np(N.nufft3d1(_t(x), _t(y), _t(z), _t(c), (10, 8, 6), eps=1e-12, isign=1)))
+    rhs = np.vdot(_np(N.nufft3d2(_t(x), _t(y), _t(z), _t(f), eps=1e-12, isign=-1)), c)
+    assert abs(lhs - rhs) < 1e-9 * abs(lhs)
+
+
+# ------------------------------------------------------------------ type 3
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_type3_vs_direct(rng, dim, prec):
+    import nufftax_amd as N
+
+    M, Nt = 2000, 1500
+    xs = [rng.uniform(-np.pi, np.pi, M) for _ in range(dim)]
+    ss = [rng.uniform(-20, 20, Nt) for _ in range(dim)]
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    eps = 1e-6 if prec == "c64" else 1e-9
+    rd, cd = (torch.float32, torch.complex64) if prec == "c64" else (torch.float64, torch.complex128)
+    if prec == "c64":
+        xs = [x.astype(np.float32).astype(np.float64) for x in xs]
+        ss = [s.astype(np.float32).astype(np.float64) for s in ss]
+        c = c.astype(np.complex64).astype(np.complex128)
+    if dim == 1:
+        nm = N.compute_type3_grid_size(xs[0], ss[0], eps=eps)
+        out = N.nufft1d3(_t(xs[0], rd), _t(c, cd), _t(ss[0], rd), nm, eps=eps)
+    elif dim == 2:
+        ext = [float((a.max() - a.min()) / 2) for a in xs + ss]
+        nm = N.compute_type3_grid_sizes_2d(ext[0], ext[1], ext[2], ext[3], eps=eps)
+        out = N.nufft2d3(_t(xs[0], rd), _t(xs[1], rd), _t(c, cd), _t(ss[0], rd), _t(ss[1], rd), nm, eps=eps)
+    else:
+        ext = [float((a.max() - a.min()) / 2) for a in xs + ss]
+        nm = N.compute_type3_grid_sizes_3d(*ext, eps=eps)
+        out = N.nufft3d3(*[_t(x, rd) for x in xs], _t(c, cd), *[_t(s, rd) for s in ss], nm, eps=eps)
+    truth = R.direct_type3(xs, c, ss)
+    err = R.rel_l2(_np(out), truth)
+    if prec == "c64":
+        nfs = nm if isinstance(nm, tuple) else (nm,)
+        proxy = R.rel_l2(R.nufft_type3([x.astype(np.float32) for x in xs], c.astype(np.complex64),
+                                       [s.astype(np.float32) for s in ss], nfs, eps=eps), truth)
+        assert err <= 2 * proxy + 1e-6, (err, proxy)
+    else:
+        assert err < 1e-7
