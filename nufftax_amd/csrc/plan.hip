@@ -102,6 +102,14 @@ std::vector<double> deconv_factors(const std::vector<double>& ph, int64_t n, int
   return f;
 }
 
+// Sort/subproblem tuning, overridable for sweeps (NUFFTX_MAXSUB, NUFFTX_SORT_CHUNK).
+SortTuning default_tuning() {
+  SortTuning t;
+  if (const char* e = std::getenv("NUFFTX_MAXSUB")) t.maxsub = std::max(64, std::atoi(e));
+  if (const char* e = std::getenv("NUFFTX_SORT_CHUNK")) t.chunk = std::max(256, std::atoi(e));
+  return t;
+}
+
 int lds_budget_bytes() {
   static int budget = [] {
     int b = 65536;
@@ -118,7 +126,8 @@ int lds_budget_bytes() {
 // Bin sizes per dimension: tiles (bin + ns halo) must fit the LDS budget and the
 // bin count the LDS histogram (16384 counters).
 void choose_bins(Geometry& g, int prec, int budget) {
-  const size_t cell = prec == kF32 ? 8 : 16;
+  (void)prec;
+  const size_t cell = 16;  // spread tiles hold two 8-byte accumulators per cell
   int bs[3];
   if (g.dim == 1) {
     bs[0] = 1024;
@@ -442,6 +451,7 @@ int nufftx_makeplan(int type, int dim, const int64_t* n_modes, int isign, int64_
   p->eps = eps;
   p->upsampfac = upsampfac;
   p->stream = static_cast<hipStream_t>(stream);
+  p->tune = default_tuning();
   kernel_params(eps, upsampfac, p->ns, p->beta, p->kc);
   p->quad = quad_rule(p->ns, p->beta, p->kc);
   p->g.dim = p->gi.dim = dim;
@@ -519,6 +529,7 @@ int nufftx_makeplan_grid(int dim, const int64_t* nf, int nspread, double beta, d
   p->beta = beta;
   p->kc = kc;
   p->stream = static_cast<hipStream_t>(stream);
+  p->tune = default_tuning();
   p->g.dim = dim;
   p->g.ns = nspread;
   for (int d = 0; d < 3; ++d) p->g.nf[d] = d < dim ? (int)nf[d] : 1;

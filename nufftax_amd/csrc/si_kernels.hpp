@@ -99,13 +99,67 @@ __device__ __forceinline__ int64_t tile_cell_to_global(int64_t cell, const int (
   return ((int64_t)g2 * nf[1] + g1) * nf[0] + g0;
 }
 
+// Rotated support walk for spreading.  Lane l visits support offset
+// (k + rot_d) mod NS at step k of dimension d, with (rot_0, rot_1) distinct over
+// the 64 lanes of a wave.  Lanes whose points share a support (dense regions,
+// e.g. the centre of a radial trajectory) then update 64 distinct cells per
+// instruction instead of serialising on one address; the weights are simply
+// evaluated at the rotated offsets, so the rotation costs no extra work.
+template <int NS>
+__device__ __forceinline__ void lane_rotation(int (&rot)[3]) {
+  const int lane = (int)(threadIdx.x & 63);
+  rot[0] = lane % NS;
+  rot[1] = (lane / NS) % NS;
+  rot[2] = (rot[0] + rot[1]) % NS;
+}
+
+template <int NS>
+__device__ __forceinline__ int rot_off(int k, int r) {
+  int kk = k + r;
+  return kk >= NS ? kk - NS : kk;
+}
+
+// rotated weights and support offsets (relative to i0) of one dimension
+template <typename T, typename CT, int NS>
+__device__ __forceinline__ void rot_weights(const SIArgs<T>& a, int d, int p, int rot, T (&w)[NS], int& i0, T& t) {
+  const double x = (double)static_cast<const CT*>(a.coord[d])[p];
+  const Folded f = fold_coord(x, a.nf[d]);
+  i0 = support_start(f, NS / 2);
+  t = (T)((double)i0 - f.xs);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) w[k] = es_phi(t + (T)rot_off<NS>(k, rot), a.es.beta_log2e, a.es.c);
+}
+
 // ------------------------------------------------------------------ spread (tiled)
+// LDS accumulation.  gfx950's LDS float atomics (ds_add_f32) run ~40x slower than
+// its integer ones, so FP32 plans accumulate each tile in exact 64-bit fixed
+// point: a contribution v becomes round(v * 2^s) through one FP64 FMA with the
+// constant 1.5 * 2^52 (the rounded integer lands in the low mantissa bits), and
+// is added with ds_add_u64.  s is chosen per subproblem from B = sum |Re c| + |Im c|
+// of its points so that every partial sum stays below 2^50; the tile sum is then
+// exact and independent of the order of the atomics.  FP64 plans use ds_add_f64.
+template <typename T>
+struct TileAcc;
+
+template <>
+struct TileAcc<float> {
+  using Word = unsigned long long;
+  static constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+  static __device__ __forceinline__ Word quant(double a, double b) {
+    const double d = __builtin_fma(a, b, kMagic);
+    return (Word)(__double_as_longlong(d) - 0x4338000000000000LL);
+  }
+};
+
 template <typename T, typename CT, int D, int NS>
 __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>* __restrict__ c,
                                                       cplx<T>* __restrict__ fw) {
+  constexpr bool kFixed = std::is_same<T, float>::value;
+  using Acc = typename std::conditional<kFixed, unsigned long long, double>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sre = reinterpret_cast<T*>(smem_raw);
-  T* sim = sre + a.tile_cells;
+  Acc* sre = reinterpret_cast<Acc*>(smem_raw);
+  Acc* sim = sre + a.tile_cells;
+  __shared__ double red[4];
   const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
   const int tr = blockIdx.y;
@@ -113,52 +167,97 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
   int org[3];
   tile_origin<D>(sd, a.bs, a.nb, NS / 2, org);
   for (int64_t i = threadIdx.x; i < a.tile_cells; i += blockDim.x) {
-    sre[i] = (T)0;
-    sim[i] = (T)0;
+    sre[i] = (Acc)0;
+    sim[i] = (Acc)0;
+  }
+  const cplx<T>* cT = c + (int64_t)tr * a.M;
+  // fixed-point scale of this subproblem
+  double scale = 1.0, inv_scale = 1.0;
+  bool finite = true;
+  if constexpr (kFixed) {
+    double part = 0.0;
+    for (int j = threadIdx.x; j < sd.z; j += blockDim.x) {
+      const int p = a.perm[sd.y + j];
+      if ((unsigned)p >= (unsigned)a.M) continue;
+      const cplx<T> cv = cT[p];
+      part += fabs((double)cv.re) + fabs((double)cv.im);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+    __syncthreads();
+    const double B = red[0] + red[1] + red[2] + red[3];
+    finite = isfinite(B);
+    if (finite && B > 0.0) {
+      int e;
+      (void)frexp(B, &e);  // B < 2^e
+      scale = ldexp(1.0, 50 - e);
+      inv_scale = ldexp(1.0, e - 50);
+    }
   }
   __syncthreads();
-  const cplx<T>* cT = c + (int64_t)tr * a.M;
   const int t0 = a.tile[0], t1 = a.tile[1];
-  for (int j = threadIdx.x; j < sd.z; j += blockDim.x) {
+  int rot[3];
+  lane_rotation<NS>(rot);
+  for (int j = threadIdx.x; j < sd.z && finite; j += blockDim.x) {
     const int p = a.perm[sd.y + j];
     if ((unsigned)p >= (unsigned)a.M) continue;
     const cplx<T> cv = cT[p];
-    T w[D][NS];
-    int i0[D];
-    point_weights<T, CT, D, NS>(a, p, w, i0);
-    const int l0 = i0[0] - org[0];
+    T w0[NS], w1[NS];
+    int i00, i01 = 0;
+    T tx, ty;
+    rot_weights<T, CT, NS>(a, 0, p, rot[0], w0, i00, tx);
+    const int l0 = i00 - org[0];
+    int o0[NS];
+    double wd[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      o0[k] = l0 + rot_off<NS>(k, rot[0]);
+      wd[k] = (double)w0[k];
+    }
+    // add (vr, vi) * w to cell `idx` of the tile
+    auto put = [&](int idx, double vr, double vi, int k) {
+      if constexpr (kFixed) {
+        atomicAdd(sre + idx, TileAcc<float>::quant(vr, wd[k]));
+        atomicAdd(sim + idx, TileAcc<float>::quant(vi, wd[k]));
+      } else {
+        atomicAdd(sre + idx, vr * wd[k]);
+        atomicAdd(sim + idx, vi * wd[k]);
+      }
+    };
+    const double cre = (double)cv.re * scale, cim = (double)cv.im * scale;
     if constexpr (D == 1) {
 #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        atomicAdd(sre + l0 + k, cv.re * w[0][k]);
-        atomicAdd(sim + l0 + k, cv.im * w[0][k]);
-      }
-    } else if constexpr (D == 2) {
-      const int l1 = i0[1] - org[1];
-#pragma unroll
-      for (int ky = 0; ky < NS; ++ky) {
-        const T vr = cv.re * w[1][ky], vi = cv.im * w[1][ky];
-        const int row = (l1 + ky) * t0 + l0;
-#pragma unroll
-        for (int kx = 0; kx < NS; ++kx) {
-          atomicAdd(sre + row + kx, vr * w[0][kx]);
-          atomicAdd(sim + row + kx, vi * w[0][kx]);
-        }
-      }
+      for (int k = 0; k < NS; ++k) put(o0[k], cre, cim, k);
     } else {
-      const int l1 = i0[1] - org[1];
-      const int l2 = i0[2] - org[2];
-#pragma unroll 1
-      for (int kz = 0; kz < NS; ++kz) {
-        const T zr = cv.re * w[2][kz], zi = cv.im * w[2][kz];
+      rot_weights<T, CT, NS>(a, 1, p, rot[1], w1, i01, ty);
+      const int l1 = i01 - org[1];
+      if constexpr (D == 2) {
 #pragma unroll
         for (int ky = 0; ky < NS; ++ky) {
-          const T vr = zr * w[1][ky], vi = zi * w[1][ky];
-          const int row = ((l2 + kz) * t1 + (l1 + ky)) * t0 + l0;
+          const double vr = cre * (double)w1[ky], vi = cim * (double)w1[ky];
+          const int row = (l1 + rot_off<NS>(ky, rot[1])) * t0;
 #pragma unroll
-          for (int kx = 0; kx < NS; ++kx) {
-            atomicAdd(sre + row + kx, vr * w[0][kx]);
-            atomicAdd(sim + row + kx, vi * w[0][kx]);
+          for (int kx = 0; kx < NS; ++kx) put(row + o0[kx], vr, vi, kx);
+        }
+      } else {
+        const double xz = (double)static_cast<const CT*>(a.coord[2])[p];
+        const Folded fz = fold_coord(xz, a.nf[2]);
+        const int i02 = support_start(fz, NS / 2);
+        const T tz = (T)((double)i02 - fz.xs);
+        const int l2 = i02 - org[2];
+#pragma unroll 1
+        for (int kz = 0; kz < NS; ++kz) {
+          const int oz = rot_off<NS>(kz, rot[2]);
+          const double wz = (double)es_phi(tz + (T)oz, a.es.beta_log2e, a.es.c);
+          const double zr = cre * wz, zi = cim * wz;
+          const int plane = (l2 + oz) * t1;
+#pragma unroll
+          for (int ky = 0; ky < NS; ++ky) {
+            const double vr = zr * (double)w1[ky], vi = zi * (double)w1[ky];
+            const int row = (plane + l1 + rot_off<NS>(ky, rot[1])) * t0;
+#pragma unroll
+            for (int kx = 0; kx < NS; ++kx) put(row + o0[kx], vr, vi, kx);
           }
         }
       }
@@ -167,8 +266,17 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
   __syncthreads();
   cplx<T>* fT = fw + (int64_t)tr * a.grid_cells;
   for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x) {
-    const T vr = sre[cell], vi = sim[cell];
-    if (vr == (T)0 && vi == (T)0) continue;
+    T vr, vi;
+    if constexpr (kFixed) {
+      const long long ir = (long long)sre[cell], ii = (long long)sim[cell];
+      if (finite && ir == 0 && ii == 0) continue;
+      vr = finite ? (T)((double)ir * inv_scale) : (T)NAN;
+      vi = finite ? (T)((double)ii * inv_scale) : (T)NAN;
+    } else {
+      vr = sre[cell];
+      vi = sim[cell];
+      if (vr == (T)0 && vi == (T)0) continue;
+    }
     const int64_t gi = tile_cell_to_global<D>(cell, a.tile, org, a.nf);
     atomic_add_global(&fT[gi].re, vr);
     atomic_add_global(&fT[gi].im, vi);
@@ -185,38 +293,52 @@ __global__ void __launch_bounds__(256) k_spread_global(SIArgs<T> a, const cplx<T
   const int p = a.perm[j];
   if ((unsigned)p >= (unsigned)a.M) return;
   const cplx<T> cv = c[(int64_t)tr * a.M + p];
-  T w[D][NS];
-  int i0[D];
-  point_weights<T, CT, D, NS>(a, p, w, i0);
+  int rot[3];
+  lane_rotation<NS>(rot);
+  T w0[NS], w1[NS];
+  int i00, i01 = 0;
+  T tx, ty;
+  rot_weights<T, CT, NS>(a, 0, p, rot[0], w0, i00, tx);
   cplx<T>* fT = fw + (int64_t)tr * a.grid_cells;
   int gx[NS];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) gx[k] = wrap_idx(i0[0] + k, a.nf[0]);
+  for (int k = 0; k < NS; ++k) gx[k] = wrap_idx(i00 + rot_off<NS>(k, rot[0]), a.nf[0]);
   if constexpr (D == 1) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      atomic_add_global(&fT[gx[k]].re, cv.re * w[0][k]);
-      atomic_add_global(&fT[gx[k]].im, cv.im * w[0][k]);
+      atomic_add_global(&fT[gx[k]].re, cv.re * w0[k]);
+      atomic_add_global(&fT[gx[k]].im, cv.im * w0[k]);
     }
   } else {
+    rot_weights<T, CT, NS>(a, 1, p, rot[1], w1, i01, ty);
+    int i02 = 0;
+    T tz = 0;
+    if constexpr (D == 3) {
+      const double xz = (double)static_cast<const CT*>(a.coord[2])[p];
+      const Folded fz = fold_coord(xz, a.nf[2]);
+      i02 = support_start(fz, NS / 2);
+      tz = (T)((double)i02 - fz.xs);
+    }
     const int kzn = D == 3 ? NS : 1;
 #pragma unroll 1
     for (int kz = 0; kz < kzn; ++kz) {
       T zr = cv.re, zi = cv.im;
       int64_t zoff = 0;
       if constexpr (D == 3) {
-        zr *= w[2][kz];
-        zi *= w[2][kz];
-        zoff = (int64_t)wrap_idx(i0[2] + kz, a.nf[2]) * a.nf[1];
+        const int oz = rot_off<NS>(kz, rot[2]);
+        const T wz = es_phi(tz + (T)oz, a.es.beta_log2e, a.es.c);
+        zr *= wz;
+        zi *= wz;
+        zoff = (int64_t)wrap_idx(i02 + oz, a.nf[2]) * a.nf[1];
       }
 #pragma unroll
       for (int ky = 0; ky < NS; ++ky) {
-        const T vr = zr * w[1][ky], vi = zi * w[1][ky];
-        const int64_t row = (zoff + wrap_idx(i0[1] + ky, a.nf[1])) * a.nf[0];
+        const T vr = zr * w1[ky], vi = zi * w1[ky];
+        const int64_t row = (zoff + wrap_idx(i01 + rot_off<NS>(ky, rot[1]), a.nf[1])) * a.nf[0];
 #pragma unroll
         for (int kx = 0; kx < NS; ++kx) {
-          atomic_add_global(&fT[row + gx[kx]].re, vr * w[0][kx]);
-          atomic_add_global(&fT[row + gx[kx]].im, vi * w[0][kx]);
+          atomic_add_global(&fT[row + gx[kx]].re, vr * w0[kx]);
+          atomic_add_global(&fT[row + gx[kx]].im, vi * w0[kx]);
         }
       }
     }
@@ -426,8 +548,10 @@ SIArgs<T> make_si_args(const GridOpArgs& ga) {
   return a;
 }
 
-inline bool use_tiles(const Geometry& g, int prec, size_t& lds) {
-  const size_t cell = prec == kF32 ? 8 : 16;
+// LDS bytes per tile cell: the spread tile holds two 8-byte accumulators per cell
+// (64-bit fixed point or FP64), the gather tile one complex value.
+inline bool use_tiles(const Geometry& g, int prec, size_t& lds, bool spread = false) {
+  const size_t cell = spread ? 16 : (prec == kF32 ? 8 : 16);
   lds = (size_t)g.tile_cells * cell;
   return lds <= (size_t)lds_budget_bytes();
 }

@@ -44,18 +44,29 @@ __device__ __forceinline__ int key_of(const KeyArgs& ka, const double (&x)[D]) {
   return key;
 }
 
+// Lanes of the calling wave whose key equals this lane's key (all lanes active),
+// from one ballot per key bit: O(log nbins), no memory traffic.
+__device__ __forceinline__ uint64_t match_key(int key, bool valid, int key_bits) {
+  uint64_t m = __ballot(valid);
+  for (int b = 0; b < key_bits; ++b) {
+    const bool bit = (key >> b) & 1;
+    const uint64_t mb = __ballot(bit);
+    m &= bit ? mb : ~mb;
+  }
+  return m;
+}
+
 constexpr int kHistThreads = 256;
 constexpr int kHistUnroll = 8;
 
 template <typename CT, int D>
 __global__ void __launch_bounds__(kHistThreads) k_hist(KeyArgs ka, int64_t M, int CH, int NB, int nbins,
-                                                       int32_t* __restrict__ hist) {
+                                                       int key_bits, int32_t* __restrict__ hist) {
   extern __shared__ int32_t cnt[];
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * CH;
   const int64_t hi = min(M, lo + (int64_t)CH);
-  const int lane = lane_id();
   for (int64_t base = lo; base < hi; base += (int64_t)kHistThreads * kHistUnroll) {
     double x[kHistUnroll][D];
 #pragma unroll
@@ -65,28 +76,21 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(KeyArgs ka, int64_t M, in
     }
 #pragma unroll
     for (int u = 0; u < kHistUnroll; ++u) {
-      int64_t i = base + (int64_t)u * kHistThreads + threadIdx.x;
-      if (i < hi) {
-        int key = key_of<D>(ka, x[u]);
-        uint64_t active = __ballot(1);
-        int k0 = __builtin_amdgcn_readfirstlane(key);
-        uint64_t same = __ballot(key == k0);
-        if (same == active) {
-          if (lane == __builtin_ctzll(active)) atomicAdd(&cnt[k0], (int)__popcll(active));
-        } else {
-          atomicAdd(&cnt[key], 1);
-        }
-      }
+      const int64_t i = base + (int64_t)u * kHistThreads + threadIdx.x;
+      const bool valid = i < hi;
+      const int key = valid ? key_of<D>(ka, x[u]) : 0;
+      const uint64_t m = match_key(key, valid, key_bits);
+      if (valid && popc_below(m) == 0) atomicAdd(&cnt[key], (int)__popcll(m));
     }
   }
   __syncthreads();
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[(int64_t)b * NB + blockIdx.x] = cnt[b];
 }
 
-constexpr int kScatUnroll = 4;
+constexpr int kScatUnroll = 8;
 
 template <typename CT, int D>
-__global__ void __launch_bounds__(64) k_scatter(KeyArgs ka, int64_t M, int CH, int NB, int nbins,
+__global__ void __launch_bounds__(64) k_scatter(KeyArgs ka, int64_t M, int CH, int NB, int nbins, int key_bits,
                                                 const int32_t* __restrict__ offs, int32_t* __restrict__ perm) {
   extern __shared__ int32_t base[];
   const int lane = threadIdx.x;
@@ -105,18 +109,12 @@ __global__ void __launch_bounds__(64) k_scatter(KeyArgs ka, int64_t M, int CH, i
     for (int u = 0; u < kScatUnroll; ++u) {
       const int64_t i = grp + u * 64 + lane;
       const bool valid = i < hi;
-      const int key = valid ? key_of<D>(ka, x[u]) : -1;
-      uint64_t pending = __ballot(valid);
-      int pos = 0;
-      while (pending) {
-        const int leader = __builtin_ctzll(pending);
-        const int lkey = __builtin_amdgcn_readlane(key, leader);
-        const uint64_t m = __ballot(key == lkey) & pending;
-        const int b0 = base[lkey];
-        if ((m >> lane) & 1ull) pos = b0 + popc_below(m);
-        if (lane == leader) base[lkey] = b0 + (int)__popcll(m);
-        pending &= ~m;
-      }
+      const int key = valid ? key_of<D>(ka, x[u]) : 0;
+      const uint64_t m = match_key(key, valid, key_bits);
+      const int rank = popc_below(m);
+      const int b0 = base[key];
+      if (valid && rank == 0) base[key] = b0 + (int)__popcll(m);
+      const int pos = b0 + rank;
       if (valid && pos >= 0 && (int64_t)pos < M) perm[pos] = (int32_t)i;
     }
   }
@@ -258,15 +256,17 @@ hipError_t grow(T*& p, int64_t& cap, int64_t need) {
 template <typename CT, int D>
 void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, hipStream_t s) {
   const size_t lds = (size_t)nbins * sizeof(int32_t);
+  int key_bits = 0;
+  while ((1 << key_bits) < nbins) ++key_bits;
   hipLaunchKernelGGL((k_hist<CT, D>), dim3(ps.NB), dim3(kHistThreads), lds, s, ka, ps.M, ps.CH, ps.NB, nbins,
-                     ps.hist);
+                     key_bits, ps.hist);
   const int64_t n = (int64_t)nbins * ps.NB;
   const int nparts = (int)((n + kScanTile - 1) / kScanTile);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kScanThreads), 0, s, ps.hist, n, ps.scan_part);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ps.scan_part, nparts);
   hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kScanThreads), 0, s, ps.hist, n, ps.scan_part);
-  hipLaunchKernelGGL((k_scatter<CT, D>), dim3(ps.NB), dim3(64), lds, s, ka, ps.M, ps.CH, ps.NB, nbins, ps.hist,
-                     ps.perm);
+  hipLaunchKernelGGL((k_scatter<CT, D>), dim3(ps.NB), dim3(64), lds, s, ka, ps.M, ps.CH, ps.NB, nbins, key_bits,
+                     ps.hist, ps.perm);
 }
 
 }  // namespace

@@ -9,7 +9,7 @@ template <typename T, typename CT, int D>
 hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
   const SIArgs<T> a = make_si_args<T>(ga);
   size_t lds = 0;
-  const bool tiled = use_tiles(*ga.g, ga.prec, lds);
+  const bool tiled = use_tiles(*ga.g, ga.prec, lds, true);
   const PointSet& ps = *ga.ps;
   return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;

@@ -227,6 +227,11 @@ def _vjp_type12(kind, meta: _Meta, src, coords, g, need_src, need_coord):
 # ---------------------------------------------------------------- autograd functions
 
 
+def _sum_lead(v):
+    """Sum over the leading (transform) axes, keeping the last one."""
+    return v.reshape(-1, v.shape[-1]).sum(0) if v.ndim > 1 else v
+
+
 def _unbatched_apply(fn, info, in_dims, args):
     """Generic vmap fallback: run per batch element and stack (primitives.py:43-46)."""
     outs = []
@@ -276,7 +281,7 @@ class _Type1(torch.autograd.Function):
         if not terms:
             return torch.zeros(c.shape[:-1] + tuple(reversed(meta.n_modes)), dtype=c.dtype, device=c.device)
         stacked = torch.stack([t.to(c.dtype) for t in terms], 0)
-        res = _raw_type1(meta, stacked, coords)
+        res = _Type1.apply(meta, stacked, *coords)
         out = torch.zeros_like(res[0])
         rdt = torch.float64 if c.dtype == torch.complex128 else torch.float32
         for r, w in zip(res, weights):
@@ -341,7 +346,7 @@ class _Type2(torch.autograd.Function):
         M = coords[0].shape[-1]
         if not srcs:
             return torch.zeros(f.shape[: f.ndim - dim] + (M,), dtype=f.dtype, device=f.device)
-        res = _raw_type2(meta, torch.stack(srcs, 0), coords)
+        res = _Type2.apply(meta, torch.stack(srcs, 0), *coords)
         out = torch.zeros_like(res[0])
         for r, m in zip(res, mults):
             out = out + (r if m is None else (1j * meta.isign) * m * r)
@@ -401,7 +406,7 @@ class _Type3(torch.autograd.Function):
                 else:
                     # dL/dx_j = Re(i isign c_j conj(sum_k g_k s_k e^{-i isign s_k x_j}))
                     v = (1j * meta.isign) * c * torch.conj(r)
-                    grads_x[tag] = v.real.sum(dim=tuple(range(v.ndim - 1))).to(xs[tag].dtype)
+                    grads_x[tag] = _sum_lead(v.real).to(xs[tag].dtype)
         # target-coordinate grads: dL/ds_k = Re(conj(g_k) i isign T3_{isign}(x -> s; x*c)_k)
         srcs2, tags2 = [], []
         for d in range(dim):
@@ -412,7 +417,7 @@ class _Type3(torch.autograd.Function):
             res2 = _raw_type3(meta, torch.stack(srcs2, 0), xs, ss)
             for r, tag in zip(res2, tags2):
                 v = torch.conj(g.to(cdt)) * (1j * meta.isign) * r
-                grads_s[tag] = v.real.sum(dim=tuple(range(v.ndim - 1))).to(ss[tag].dtype)
+                grads_s[tag] = _sum_lead(v.real).to(ss[tag].dtype)
         if gc is not None:
             gc = gc.to(c.dtype)
         return (None, gc) + tuple(grads_x) + tuple(grads_s)
@@ -440,7 +445,7 @@ class _Type3(torch.autograd.Function):
         N = ss[0].shape[-1]
         if not srcs:
             return torch.zeros(c.shape[:-1] + (N,), dtype=c.dtype, device=c.device)
-        res = _raw_type3(meta, torch.stack(srcs, 0), xs, ss)
+        res = _Type3.apply(meta, torch.stack(srcs, 0), *xs, *ss)
         out = torch.zeros_like(res[0])
         for r, p in zip(res, post):
             if p is None:
