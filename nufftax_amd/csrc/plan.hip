@@ -126,15 +126,16 @@ int lds_budget_bytes() {
 // Bin sizes per dimension: tiles (bin + ns halo) must fit the LDS budget and the
 // bin count the LDS histogram (16384 counters).
 void choose_bins(Geometry& g, int prec, int budget) {
-  (void)prec;
   const size_t cell = 16;  // spread tiles hold two 8-byte accumulators per cell
   int bs[3];
   if (g.dim == 1) {
     bs[0] = 1024;
     bs[1] = bs[2] = 1;
   } else if (g.dim == 2) {
-    bs[0] = 32;
-    bs[1] = 32;
+    // FP32, ns <= 8: bins of 4 x 4 sub-bins of 16 - ns cells for the MFMA spread
+    const int b2 = (prec == kF32 && g.ns <= 8) ? 4 * (16 - g.ns) : 32;
+    bs[0] = b2;
+    bs[1] = b2;
     bs[2] = 1;
   } else {
     bs[0] = 16;

@@ -1,5 +1,7 @@
 // Type-1 spreading launchers (kernels in si_kernels.hpp).
-#include "si_kernels.hpp"
+#include <cstdlib>
+
+#include "si_mfma.hpp"
 
 namespace nufftx {
 
@@ -27,8 +29,34 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
   });
 }
 
+template <typename CT>
+hipError_t spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
+  const SIArgs<float> a = make_si_args<float>(ga);
+  const PointSet& ps = *ga.ps;
+  auto launch = [&](auto nsc) -> hipError_t {
+    constexpr int NS = decltype(nsc)::value;
+    const size_t lds = MfmaGeom<NS>::lds_bytes();
+    auto kern = k_spread_mfma2d<CT, NS>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(64 * kMfmaWaves), lds, s, a,
+                       static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
+    return hipGetLastError();
+  };
+  switch (ga.g->ns) {
+    case 2: return launch(std::integral_constant<int, 2>{});
+    case 4: return launch(std::integral_constant<int, 4>{});
+    case 6: return launch(std::integral_constant<int, 6>{});
+    case 8: return launch(std::integral_constant<int, 8>{});
+  }
+  return hipErrorInvalidValue;
+}
+
 template <typename T, typename CT>
 hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (mfma2d_applicable(*ga.g, ga.prec) && std::getenv("NUFFTX_NO_MFMA") == nullptr)
+      return spread_mfma<CT>(ga, c, fw, s);
+  }
   switch (ga.g->dim) {
     case 1: return spread_impl<T, CT, 1>(ga, c, fw, s);
     case 2: return spread_impl<T, CT, 2>(ga, c, fw, s);

@@ -1,0 +1,127 @@
+"""CPU checks of the C-ABI library and the host-side logic (no GPU compute calls)."""
+
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import nufft_ref as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nufftx.h")
+
+
+def _declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nufftx_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from nufftax_amd import _lib
+
+    handle = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _declared_symbols()
+    assert len(declared) >= 15
+    missing = [s for s in declared if not hasattr(handle, s)]
+    assert not missing, missing
+    # the Python binding covers the whole header, with explicit signatures
+    assert set(declared) == set(_lib.SIGNATURES), set(declared) ^ set(_lib.SIGNATURES)
+    _lib.lib()  # binds every signature
+
+
+def test_version_and_errors_without_gpu():
+    from nufftax_amd import _lib
+
+    L = _lib.lib()
+    assert L.nufftx_version() == 100
+    h = ctypes.c_void_p()
+    nm = (ctypes.c_int64 * 3)(16, 16, 1)
+    # argument validation happens before any device call
+    assert L.nufftx_makeplan(7, 2, nm, 1, 1, 1e-6, 2.0, 0, 0, None, ctypes.byref(h)) == 1
+    assert "type" in _lib.last_error()
+    assert L.nufftx_makeplan(1, 4, nm, 1, 1, 1e-6, 2.0, 0, 0, None, ctypes.byref(h)) == 1
+    assert L.nufftx_makeplan(1, 2, nm, 1, 0, 1e-6, 2.0, 0, 0, None, ctypes.byref(h)) == 1
+    assert L.nufftx_makeplan(1, 2, nm, 1, 1, -1.0, 2.0, 0, 0, None, ctypes.byref(h)) == 1
+    assert L.nufftx_makeplan(1, 2, nm, 1, 1, 1e-6, 0.9, 0, 0, None, ctypes.byref(h)) == 1
+    assert L.nufftx_makeplan(1, 2, nm, 1, 1, 1e-6, 2.0, 5, 0, None, ctypes.byref(h)) == 1
+    assert L.nufftx_setpts(None, 10, None, None, None, 0, None, None, None) == 1
+    assert L.nufftx_execute(None, None, None) == 1
+    assert L.nufftx_destroy(None) == 0
+    assert L.nufftx_grid_size(-1, 2.0, 8) < 0
+
+
+@pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6, 1e-9, 1e-12, 1e-15])
+@pytest.mark.parametrize("upsampfac", [2.0, 1.25, 1.5])
+def test_kernel_params_match_oracle(eps, upsampfac):
+    import nufftax_amd as N
+
+    kp = N.compute_kernel_params(eps, upsampfac)
+    ref = R.compute_kernel_params(eps, upsampfac)
+    assert kp.nspread == ref.nspread and kp.beta == pytest.approx(ref.beta, rel=1e-15) and kp.c == ref.c
+    for n in (1, 7, 32, 100, 512, 1000, 1023):
+        assert N.compute_grid_size(n, upsampfac, kp.nspread) == R.compute_grid_size(n, upsampfac, kp.nspread)
+
+
+def test_type3_grid_helpers_match_oracle(rng):
+    import nufftax_amd as N
+
+    x = rng.uniform(-3, 1, 100)
+    s = rng.uniform(-40, 10, 80)
+    assert N.compute_type3_grid_size(x, s, eps=1e-9) == R.compute_type3_grid_size(x, s, eps=1e-9)
+    assert N.compute_type3_grid_size(math.pi, 64.0, eps=1e-9) == 270
+    assert N.compute_type3_grid_sizes_2d(math.pi, 1.0, 64.0, 5.0, eps=1e-6) == (
+        R.compute_type3_grid_size(math.pi, 64.0, eps=1e-6), R.compute_type3_grid_size(1.0, 5.0, eps=1e-6))
+    nf3 = N.compute_type3_grid_sizes_3d(1, 2, 3, 4, 5, 6, eps=1e-4)
+    assert all(v % 2 == 0 for v in nf3)
+
+
+def test_public_api_surface():
+    import nufftax_amd as N
+
+    for name in ["nufft1d1", "nufft1d2", "nufft1d3", "nufft2d1", "nufft2d2", "nufft2d3", "nufft3d1", "nufft3d2",
+                 "nufft3d3", "compute_type3_grid_size", "compute_type3_grid_sizes_2d", "compute_type3_grid_sizes_3d",
+                 "fftc", "ifftc", "to_complex", "from_complex"]:
+        assert callable(getattr(N, name))
+        assert callable(getattr(N, name + "_jvp", getattr(N, name)))
+
+
+def test_centered_fft_utils_match_numpy(rng):
+    import torch
+
+    import nufftax_amd as N
+
+    a = rng.standard_normal((6, 8)) + 1j * rng.standard_normal((6, 8))
+    t = torch.from_numpy(a)
+    ref = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(a)))
+    np.testing.assert_allclose(N.fftc(t).numpy(), ref, rtol=1e-12)
+    np.testing.assert_allclose(N.ifftc(N.fftc(t)).numpy(), a, rtol=1e-12, atol=1e-12)
+    two = N.from_complex(t)
+    assert two.shape == (6, 8, 2)
+    np.testing.assert_allclose(N.to_complex(two).numpy(), a)
+
+
+def test_no_cpu_fallback():
+    """Without a GPU the transforms raise instead of computing on the CPU."""
+    import torch
+
+    import nufftax_amd as N
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(Exception):
+        N.nufft1d1(np.zeros(4), np.ones(4, complex), 8)
+
+
+def test_workload_generators_are_seeded():
+    from nufftax_amd import workloads
+
+    a = workloads.radial_2d(seed=3, spokes=8, samples=10, n_modes=(4, 4))
+    b = workloads.radial_2d(seed=3, spokes=8, samples=10, n_modes=(4, 4))
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+    x, y, c, f = a
+    assert x.shape == (80,) and f.shape == (4, 4) and np.all(np.abs(x) <= np.pi) and np.all(np.abs(y) <= np.pi)
