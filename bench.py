@@ -192,10 +192,13 @@ def main():
     nfc = nf[0] * nf[1]
     bytes_spread = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c gathered; grid written once
     bytes_interp = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c written; grid read once
+    mfma = os.environ.get("NUFFTX_NO_MFMA") is None  # the 2D FP32 ns <= 8 path runs on the matrix cores
     if t_spread >= t_interp:
-        kname, kt, kb = "k_spread_tiled<float,float,2,8>", t_spread, bytes_spread
+        kname = "k_spread_mfma2d<float,8>" if mfma else "k_spread_tiled<float,float,2,8>"
+        kt, kb = t_spread, bytes_spread
     else:
-        kname, kt, kb = "k_gather_tiled<float,float,2,8,false>", t_interp, bytes_interp
+        kname = "k_interp_mfma2d<float,8>" if mfma else "k_gather_tiled<float,float,2,8,false>"
+        kt, kb = t_interp, bytes_interp
     achieved = kb / kt / 1e9
 
     # accuracy against the FP64 direct NUDFT on subsets

@@ -149,6 +149,26 @@ int nufftx_interp(nufftx_plan plan, const void* fw, void* c);
 int nufftx_gather_grad(nufftx_plan plan, const void* G, const void* P, void* gval, void* grad_x, void* grad_y,
                        void* grad_z);
 
+/*
+ * phihat[k], k = 0 .. nf/2, of the ES kernel (nspread, beta, c) by the FP64 trapezoid
+ * rule the plans use (kernel_fourier_series, nufftax/core/kernel.py:209-260).
+ * Host memory, out has nf/2 + 1 entries.
+ */
+int nufftx_kernel_fourier_series(int64_t nf, int nspread, double beta, double c, double* out);
+
+/*
+ * Deconvolution with mode truncation/padding on device buffers, enqueued on stream:
+ *   direction 0: out[t][modes] = in[t][fine grid] * prod_d (-1)^k_d / phihat_d[|k_d|]
+ *                (deconvolve_shuffle_{1,2,3}d, nufftax/core/deconvolve.py:81-111, 174-224, 312-366)
+ *   direction 1: out[t][fine grid] = zero grid with the deconvolved in[t][modes] in FFT order
+ *                (deconvolve_pad_{1,2,3}d, deconvolve.py:114-166, 227-304, 369-468)
+ * phihat_d: device arrays of at least n_modes[d]/2 + 1 reals of the given precision.
+ * Mode arrays are (n_trans, n3, n2, n1), grids (n_trans, nf3, nf2, nf1), x fastest.
+ */
+int nufftx_deconvolve(int direction, int dim, const int64_t* n_modes, const int64_t* nf, int64_t n_trans,
+                      const void* phihat1, const void* phihat2, const void* phihat3, int modeord, int precision,
+                      void* stream, const void* in, void* out);
+
 /* Bin geometry of a plan (inner != 0: the inner grid of a type-3 plan). */
 int nufftx_get_sort_info(nufftx_plan plan, int inner, int* nbins, int* bin_size /*[3]*/,
                          int* n_bins_per_dim /*[3]*/);

@@ -2,7 +2,8 @@
 
 Drop-in for the public API of nufftax (nufftax/__init__.py:55-99): the nine
 transforms, the type-3 grid-size helpers and the centered-FFT utilities, on
-PyTorch tensors.  The compute path is the nufftx HIP library (libnufftx.so,
+PyTorch tensors; ``nufftax_amd.core`` mirrors ``nufftax.core`` (spread /
+interp with their VJPs, deconvolution, kernel helpers).  The compute path is the nufftx HIP library (libnufftx.so,
 C ABI in include/nufftx.h); there is no CPU fallback.
 """
 
@@ -38,6 +39,7 @@ from .transforms import (  # noqa: E402
     nufft3d3_jvp,
 )
 from .utils import fftc, from_complex, ifftc, to_complex  # noqa: E402
+from . import core  # noqa: E402
 
 __all__ = [
     "nufft1d1", "nufft2d1", "nufft3d1",
@@ -48,5 +50,5 @@ __all__ = [
     "nufft3d1_jvp", "nufft3d2_jvp", "nufft3d3_jvp",
     "compute_type3_grid_size", "compute_type3_grid_sizes_2d", "compute_type3_grid_sizes_3d",
     "compute_kernel_params", "compute_grid_size", "next_smooth_int", "KernelParams",
-    "fftc", "ifftc", "to_complex", "from_complex",
+    "fftc", "ifftc", "to_complex", "from_complex", "core",
 ]

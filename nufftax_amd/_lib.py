@@ -60,6 +60,10 @@ SIGNATURES = {
         [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     ),
     "nufftx_copy_sort": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "nufftx_kernel_fourier_series": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                    _vp]),
+    "nufftx_deconvolve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int64_p, _c_int64_p, ctypes.c_int64, _vp, _vp, _vp,
+                                         ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -57,10 +57,22 @@ struct DK {
   int nf[3];
   int nm[3];
   int modeord;
-  const T* fac[3];
+  const T* fac[3];  // factors in output mode order, or (PH) phihat[0 .. n/2]
 };
 
-template <typename T, int D>
+// Deconvolution factor of mode index m on axis d: a table lookup, or (PH) the
+// factor (-1)^k / phihat[|k|] of that mode (deconvolve.py:27-56) derived in place.
+template <bool PH, typename T>
+__device__ __forceinline__ T dfac(const DK<T>& a, int d, int m) {
+  if (!PH) return a.fac[d][m];
+  const int n = a.nm[d];
+  const int k = a.modeord == 0 ? m - n / 2 : (m <= (n - 1) / 2 ? m : m - n);
+  const int ak = k < 0 ? -k : k;
+  const T inv = (T)1 / a.fac[d][ak];
+  return (ak & 1) ? -inv : inv;
+}
+
+template <typename T, int D, bool PH>
 __global__ void __launch_bounds__(256) k_deconv_shuffle(DK<T> a, const cplx<T>* __restrict__ in,
                                                         cplx<T>* __restrict__ out) {
   const int64_t nm = (int64_t)a.nm[0] * a.nm[1] * a.nm[2];
@@ -76,21 +88,21 @@ __global__ void __launch_bounds__(256) k_deconv_shuffle(DK<T> a, const cplx<T>* 
     if (D >= 3) r /= a.nm[2];
     const int64_t tr = r;
     int64_t gidx = mode_to_grid(m0, a.nm[0], a.nf[0], a.modeord);
-    T fac = a.fac[0][m0];
+    T fac = dfac<PH>(a, 0, m0);
     if (D >= 2) {
       gidx += (int64_t)mode_to_grid(m1, a.nm[1], a.nf[1], a.modeord) * a.nf[0];
-      fac *= a.fac[1][m1];
+      fac *= dfac<PH>(a, 1, m1);
     }
     if (D >= 3) {
       gidx += (int64_t)mode_to_grid(m2, a.nm[2], a.nf[2], a.modeord) * a.nf[0] * a.nf[1];
-      fac *= a.fac[2][m2];
+      fac *= dfac<PH>(a, 2, m2);
     }
     const cplx<T> v = in[tr * ng + gidx];
     out[idx] = {v.re * fac, v.im * fac};
   }
 }
 
-template <typename T, int D>
+template <typename T, int D, bool PH>
 __global__ void __launch_bounds__(256) k_deconv_pad(DK<T> a, const cplx<T>* __restrict__ in,
                                                     cplx<T>* __restrict__ out) {
   const int64_t nm = (int64_t)a.nm[0] * a.nm[1] * a.nm[2];
@@ -110,14 +122,14 @@ __global__ void __launch_bounds__(256) k_deconv_pad(DK<T> a, const cplx<T>* __re
     const int m1 = D >= 2 ? grid_to_mode(g1, a.nm[1], a.nf[1], a.modeord) : 0;
     const int m2 = D >= 3 ? grid_to_mode(g2, a.nm[2], a.nf[2], a.modeord) : 0;
     if (m0 >= 0 && m1 >= 0 && m2 >= 0) {
-      T fac = a.fac[0][m0];
+      T fac = dfac<PH>(a, 0, m0);
       int64_t midx = m0;
       if (D >= 2) {
-        fac *= a.fac[1][m1];
+        fac *= dfac<PH>(a, 1, m1);
         midx += (int64_t)m1 * a.nm[0];
       }
       if (D >= 3) {
-        fac *= a.fac[2][m2];
+        fac *= dfac<PH>(a, 2, m2);
         midx += (int64_t)m2 * a.nm[0] * a.nm[1];
       }
       const cplx<T> f = in[tr * nm + midx];
@@ -148,9 +160,17 @@ hipError_t shuffle_t(const DeconvArgs& a, const void* in, void* out, hipStream_t
   const unsigned g = grid_for(total);
   auto i = static_cast<const cplx<T>*>(in);
   auto o = static_cast<cplx<T>*>(out);
-  if (a.dim == 1) hipLaunchKernelGGL((k_deconv_shuffle<T, 1>), dim3(g), dim3(256), 0, s, k, i, o);
-  if (a.dim == 2) hipLaunchKernelGGL((k_deconv_shuffle<T, 2>), dim3(g), dim3(256), 0, s, k, i, o);
-  if (a.dim == 3) hipLaunchKernelGGL((k_deconv_shuffle<T, 3>), dim3(g), dim3(256), 0, s, k, i, o);
+  const bool ph = a.from_phihat;
+  if (a.dim == 1) {
+    if (ph) hipLaunchKernelGGL((k_deconv_shuffle<T, 1, true>), dim3(g), dim3(256), 0, s, k, i, o);
+    else hipLaunchKernelGGL((k_deconv_shuffle<T, 1, false>), dim3(g), dim3(256), 0, s, k, i, o);
+  } else if (a.dim == 2) {
+    if (ph) hipLaunchKernelGGL((k_deconv_shuffle<T, 2, true>), dim3(g), dim3(256), 0, s, k, i, o);
+    else hipLaunchKernelGGL((k_deconv_shuffle<T, 2, false>), dim3(g), dim3(256), 0, s, k, i, o);
+  } else {
+    if (ph) hipLaunchKernelGGL((k_deconv_shuffle<T, 3, true>), dim3(g), dim3(256), 0, s, k, i, o);
+    else hipLaunchKernelGGL((k_deconv_shuffle<T, 3, false>), dim3(g), dim3(256), 0, s, k, i, o);
+  }
   return hipGetLastError();
 }
 
@@ -162,9 +182,17 @@ hipError_t pad_t(const DeconvArgs& a, const void* in, void* out, hipStream_t s) 
   const unsigned g = grid_for(total);
   auto i = static_cast<const cplx<T>*>(in);
   auto o = static_cast<cplx<T>*>(out);
-  if (a.dim == 1) hipLaunchKernelGGL((k_deconv_pad<T, 1>), dim3(g), dim3(256), 0, s, k, i, o);
-  if (a.dim == 2) hipLaunchKernelGGL((k_deconv_pad<T, 2>), dim3(g), dim3(256), 0, s, k, i, o);
-  if (a.dim == 3) hipLaunchKernelGGL((k_deconv_pad<T, 3>), dim3(g), dim3(256), 0, s, k, i, o);
+  const bool ph = a.from_phihat;
+  if (a.dim == 1) {
+    if (ph) hipLaunchKernelGGL((k_deconv_pad<T, 1, true>), dim3(g), dim3(256), 0, s, k, i, o);
+    else hipLaunchKernelGGL((k_deconv_pad<T, 1, false>), dim3(g), dim3(256), 0, s, k, i, o);
+  } else if (a.dim == 2) {
+    if (ph) hipLaunchKernelGGL((k_deconv_pad<T, 2, true>), dim3(g), dim3(256), 0, s, k, i, o);
+    else hipLaunchKernelGGL((k_deconv_pad<T, 2, false>), dim3(g), dim3(256), 0, s, k, i, o);
+  } else {
+    if (ph) hipLaunchKernelGGL((k_deconv_pad<T, 3, true>), dim3(g), dim3(256), 0, s, k, i, o);
+    else hipLaunchKernelGGL((k_deconv_pad<T, 3, false>), dim3(g), dim3(256), 0, s, k, i, o);
+  }
   return hipGetLastError();
 }
 

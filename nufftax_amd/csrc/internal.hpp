@@ -32,7 +32,7 @@ struct PointSet {
   const void* coord[3] = {nullptr, nullptr, nullptr};
   bool coord_f64 = false;
   // sort state
-  int NB = 0;        // sort chunks (columns of the histogram matrix)
+  int NB = 0;        // sort chunks (rows of the histogram matrix)
   int CH = 0;        // points per chunk
   int maxsub = 0;    // max points per subproblem
   int max_subprob = 0;
@@ -47,7 +47,7 @@ struct PointSet {
 };
 
 struct SortTuning {
-  int chunk = 2048;            // points per histogram column
+  int chunk = 8192;            // points per histogram row (sort chunk)
   int64_t hist_cap = 8 << 20;  // max entries of the histogram matrix
   int maxsub = 1024;           // points per subproblem
 };
@@ -89,7 +89,8 @@ struct DeconvArgs {
   int nf[3];
   int nm[3];
   int modeord;
-  const void* fac[3];  // per dim, T, in output mode order
+  const void* fac[3];  // per dim, T, in output mode order; from_phihat: phihat[0 .. n/2] instead
+  bool from_phihat = false;
 };
 hipError_t launch_deconv_shuffle(const DeconvArgs& a, const void* fw_hat, void* f, hipStream_t s);
 hipError_t launch_deconv_pad(const DeconvArgs& a, const void* f, void* fw_hat, hipStream_t s);

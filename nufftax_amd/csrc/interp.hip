@@ -1,5 +1,7 @@
-// Type-2 interpolation launchers (kernels in si_kernels.hpp).
-#include "si_kernels.hpp"
+// Type-2 interpolation launchers (kernels in si_kernels.hpp and si_mfma.hpp).
+#include <cstdlib>
+
+#include "si_mfma.hpp"
 
 namespace nufftx {
 
@@ -31,8 +33,32 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
   });
 }
 
+template <typename CT>
+hipError_t interp_mfma(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
+  const SIArgs<float> a = make_si_args<float>(ga);
+  const PointSet& ps = *ga.ps;
+  auto launch = [&](auto nsc) -> hipError_t {
+    constexpr int NS = decltype(nsc)::value;
+    const size_t lds = InterpGeom<NS>::lds_bytes();
+    hipLaunchKernelGGL((k_interp_mfma2d<CT, NS>), dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kInterpThreads), lds,
+                       s, a, static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));
+    return hipGetLastError();
+  };
+  switch (ga.g->ns) {
+    case 2: return launch(std::integral_constant<int, 2>{});
+    case 4: return launch(std::integral_constant<int, 4>{});
+    case 6: return launch(std::integral_constant<int, 6>{});
+    case 8: return launch(std::integral_constant<int, 8>{});
+  }
+  return hipErrorInvalidValue;
+}
+
 template <typename T, typename CT>
 hipError_t interp_dim(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (mfma2d_applicable(*ga.g, ga.prec) && std::getenv("NUFFTX_NO_MFMA") == nullptr)
+      return interp_mfma<CT>(ga, fw, c, s);
+  }
   switch (ga.g->dim) {
     case 1: return interp_impl<T, CT, 1>(ga, fw, c, s);
     case 2: return interp_impl<T, CT, 2>(ga, fw, c, s);

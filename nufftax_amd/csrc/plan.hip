@@ -725,4 +725,48 @@ int nufftx_copy_sort(nufftx_plan p, int inner, int32_t* perm, int32_t* bin_start
   return NUFFTX_OK;
 }
 
+int nufftx_kernel_fourier_series(int64_t nf, int nspread, double beta, double c, double* out) {
+  if (nf < 1 || nspread < 2 || nspread > 16 || !out) return fail(NUFFTX_ERR_INVALID_ARG, "bad kernel Fourier series arguments");
+  const std::vector<double> ph = phihat(nf, quad_rule(nspread, beta, c));
+  std::copy(ph.begin(), ph.end(), out);
+  return NUFFTX_OK;
+}
+
+int nufftx_deconvolve(int direction, int dim, const int64_t* n_modes, const int64_t* nf, int64_t n_trans,
+                      const void* phihat1, const void* phihat2, const void* phihat3, int modeord, int precision,
+                      void* stream, const void* in, void* out) {
+  if (direction != 0 && direction != 1) return fail(NUFFTX_ERR_INVALID_ARG, "direction must be 0 (shuffle) or 1 (pad)");
+  if (dim < 1 || dim > 3 || !n_modes || !nf) return fail(NUFFTX_ERR_INVALID_ARG, "dim must be 1, 2 or 3");
+  if (precision != NUFFTX_F32 && precision != NUFFTX_F64) return fail(NUFFTX_ERR_INVALID_ARG, "bad precision");
+  if (modeord != 0 && modeord != 1) return fail(NUFFTX_ERR_INVALID_ARG, "modeord must be 0 or 1");
+  if (n_trans < 0) return fail(NUFFTX_ERR_INVALID_ARG, "n_trans must be >= 0");
+  const void* ph[3] = {phihat1, phihat2, phihat3};
+  DeconvArgs a;
+  a.dim = dim;
+  a.prec = precision == NUFFTX_F32 ? kF32 : kF64;
+  a.n_trans = n_trans;
+  a.modeord = modeord;
+  a.from_phihat = true;
+  for (int d = 0; d < 3; ++d) {
+    a.nf[d] = 1;
+    a.nm[d] = 1;
+    a.fac[d] = nullptr;
+    if (d >= dim) continue;
+    if (n_modes[d] < 1 || nf[d] < n_modes[d] || nf[d] > INT32_MAX)
+      return fail(NUFFTX_ERR_INVALID_ARG, "need 1 <= n_modes <= nf < 2^31 on every axis");
+    if (!ph[d]) return fail(NUFFTX_ERR_INVALID_ARG, "null phihat pointer");
+    a.nf[d] = (int)nf[d];
+    a.nm[d] = (int)n_modes[d];
+    a.fac[d] = ph[d];
+  }
+  if (n_trans == 0) return NUFFTX_OK;
+  if (!in || !out) return fail(NUFFTX_ERR_INVALID_ARG, "null data pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (direction == 0)
+    LAUNCH(launch_deconv_shuffle(a, in, out, s));
+  else
+    LAUNCH(launch_deconv_pad(a, in, out, s));
+  return NUFFTX_OK;
+}
+
 }  // extern "C"

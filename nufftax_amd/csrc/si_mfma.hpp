@@ -29,6 +29,13 @@ namespace nufftx {
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
+// Orders LDS accesses of different lanes of one wave: the hardware runs a wave's
+// LDS instructions in order, the fence keeps the compiler from reordering them.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int kMfmaWaves = 2;
 
 template <int NS>
@@ -127,6 +134,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
       }
       r[3 * NS] = __int_as_float(offs);
     }
+    wave_lds_fence();
     // ---- 4 points per MFMA into the sub-bin accumulators
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
@@ -147,18 +155,28 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
         acc_im[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, b, acc_im[s], 0, 0, 0);
       }
     }
+    wave_lds_fence();  // this chunk's record reads before the next chunk's writes
   }
-  // ---- accumulators -> this wave's tile (windows overlap; one wave, in order)
+  // ---- accumulators -> this wave's tile.  Windows of neighbouring sub-bins overlap,
+  // so lanes of one wave would update the same cells; the windows are added in 4
+  // phases by (sx, sy) parity -- within a phase they are disjoint (2 SB >= 16) --
+  // with a wavefront fence between phases so the compiler cannot move one phase's
+  // LDS reads above the previous phase's writes (LDS ops of a wave run in order).
   const int col = lane & 15, row0 = 4 * (lane >> 4);
 #pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int x = (s & 3) * SB + col;
+  for (int ph = 0; ph < 4; ++ph) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int y = (s >> 2) * SB + row0 + r;
-      tre[y * T + x] += acc_re[s][r];
-      tim[y * T + x] += acc_im[s][r];
+    for (int s = 0; s < 16; ++s) {
+      if (((s & 1) | (((s >> 2) & 1) << 1)) != ph) continue;
+      const int x = (s & 3) * SB + col;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int y = (s >> 2) * SB + row0 + r;
+        tre[y * T + x] += acc_re[s][r];
+        tim[y * T + x] += acc_im[s][r];
+      }
     }
+    wave_lds_fence();
   }
   __syncthreads();
   // ---- sum the wave tiles, flush to the fine grid
@@ -178,8 +196,186 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
   }
 }
 
+// 2D FP32 interpolation on the matrix cores, same bin / sub-bin geometry.
+//
+// c_j = sum_y wy_j[y] sum_x T[y][x] wx_j[x]                  (nufftax/core/spread.py:497-540)
+// For 16 points of one sub-bin the inner sum is a (16 rows x 16 cols) x (16 cols x
+// 16 points) product, D = T_win Wx, done as 4-column steps of v_mfma_f32_16x16x4f32
+// (A = window columns, B = x-weights of the 16 points); each lane then folds its 4
+// rows of D with the y-weights and 2 cross-lane adds finish the sum over y.
+//
+// A block takes one subproblem: the bin's tile (bin + halo) is loaded into LDS
+// once; points are folded (FP64) and counting-sorted in LDS by (sub-bin, x offset)
+// in segments of kInterpSeg, so every group of 16 points shares a window and a
+// narrow range of x offsets: column steps outside the group's support are skipped.
+constexpr int kInterpSeg = 1024;
+constexpr int kInterpThreads = 256;
+
+template <int NS>
+struct InterpGeom {
+  static constexpr int SB = 16 - NS;
+  static constexpr int T = 4 * SB + NS;
+  static constexpr int TP = T + 1;          // odd LDS row stride (bank spread of the A reads)
+  static constexpr int TCP = T * TP;
+  static constexpr int NKEY = 16 * (SB + 1);
+  static constexpr int MAXG = kInterpSeg / 16 + 16;
+  static constexpr size_t lds_bytes() { return 2 * (size_t)TCP * sizeof(float) + (size_t)kInterpSeg * sizeof(int4); }
+};
+
+template <typename CT, int NS>
+__global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> a, const cplx<float>* __restrict__ fw,
+                                                                   cplx<float>* __restrict__ out) {
+  using G = InterpGeom<NS>;
+  constexpr int SB = G::SB, T = G::T, TP = G::TP, TCP = G::TCP, NKEY = G::NKEY;
+  constexpr int PPT = kInterpSeg / kInterpThreads;
+  extern __shared__ __align__(16) float smem_f[];
+  __shared__ int cnt[NKEY + 1];
+  __shared__ int2 grp[G::MAXG];
+  __shared__ int ngrp;
+  float* tre = smem_f;
+  float* tim = tre + TCP;
+  int4* rec = reinterpret_cast<int4*>(tim + TCP);
+
+  const int sub = blockIdx.x;
+  if (sub >= *a.nsub) return;
+  const int tr = blockIdx.y;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int4 sd = a.subprob[sub];
+  int org[3];
+  tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
+  const int bin_x0 = org[0] + NS / 2, bin_y0 = org[1] + NS / 2;
+
+  // ---- tile (bin + halo) -> LDS
+  const cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
+  const int tile3[3] = {T, T, 1};
+  for (int cell = tid; cell < T * T; cell += kInterpThreads) {
+    const int64_t gi = tile_cell_to_global<2>(cell, tile3, org, a.nf);
+    const cplx<float> v = fT[gi];
+    const int l = (cell / T) * TP + cell % T;
+    tre[l] = v.re;
+    tim[l] = v.im;
+  }
+  cplx<float>* oT = out + (int64_t)tr * a.M;
+  const int j = lane & 15, q = lane >> 4;
+
+  for (int seg0 = 0; seg0 < sd.z; seg0 += kInterpSeg) {
+    for (int k = tid; k <= NKEY; k += kInterpThreads) cnt[k] = 0;
+    __syncthreads();
+    // ---- fold, sub-bin, key; per-key slot by LDS atomics
+    int4 r[PPT];
+    int key[PPT], slot[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+      const int jj = seg0 + u * kInterpThreads + tid;
+      key[u] = -1;
+      if (jj < sd.z) {
+        const int p = a.perm[sd.y + jj];
+        if ((unsigned)p < (unsigned)a.M) {
+          const Folded fx = fold_coord((double)static_cast<const CT*>(a.coord[0])[p], a.nf[0]);
+          const Folded fy = fold_coord((double)static_cast<const CT*>(a.coord[1])[p], a.nf[1]);
+          const int i0x = support_start(fx, NS / 2), i0y = support_start(fy, NS / 2);
+          const int sx = (fx.cell - bin_x0) / SB, sy = (fy.cell - bin_y0) / SB;
+          const int ox = (i0x - org[0]) - sx * SB, oy = (i0y - org[1]) - sy * SB;
+          const int s = sy * 4 + sx;
+          r[u] = make_int4(__float_as_int((float)((double)i0x - fx.xs)), __float_as_int((float)((double)i0y - fy.xs)),
+                           ox | (oy << 8) | (s << 16), p);
+          key[u] = s * (SB + 1) + ox;
+          slot[u] = atomicAdd(&cnt[key[u]], 1);
+        }
+      }
+    }
+    __syncthreads();
+    // ---- exclusive scan of the key counts (wave 0), then the group list (16 lanes)
+    if (wave == 0) {
+      constexpr int KPL = (NKEY + 63) / 64;
+      int v[KPL], sum = 0;
+#pragma unroll
+      for (int t = 0; t < KPL; ++t) {
+        const int k = lane * KPL + t;
+        v[t] = k < NKEY ? cnt[k] : 0;
+        sum += v[t];
+      }
+      int incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+      }
+      int run = incl - sum;
+#pragma unroll
+      for (int t = 0; t < KPL; ++t) {
+        const int k = lane * KPL + t;
+        if (k < NKEY) cnt[k] = run;
+        run += v[t];
+      }
+      if (lane == 63) cnt[NKEY] = incl;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      int st = 0, n = 0, ng = 0;
+      if (lane < 16) {
+        st = cnt[lane * (SB + 1)];
+        n = cnt[(lane + 1) * (SB + 1)] - st;
+        ng = (n + 15) >> 4;
+      }
+      int incl = ng;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+      }
+      if (lane < 16) {
+        const int g0 = incl - ng;
+        for (int g = 0; g < ng; ++g) grp[g0 + g] = make_int2(st + 16 * g, lane | (min(16, n - 16 * g) << 8));
+      }
+      if (lane == 15) ngrp = incl;
+    }
+    // ---- scatter records in key order
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+      if (key[u] >= 0) rec[cnt[key[u]] + slot[u]] = r[u];
+    __syncthreads();
+    // ---- groups of 16 points: D = T_win Wx on the matrix cores, then the y sum
+    const int ng_total = ngrp;
+    for (int g = wave; g < ng_total; g += kInterpThreads / 64) {
+      const int2 gd = grp[g];
+      const int s = gd.y & 255, gn = gd.y >> 8;
+      const bool valid = j < gn;
+      const int4 rv = rec[gd.x + min(j, gn - 1)];
+      const float tx = __int_as_float(rv.x), ty = __int_as_float(rv.y);
+      const int ox = rv.z & 255, oy = (rv.z >> 8) & 255;
+      const int ox_lo = __shfl(ox, 0, 64), ox_hi = __shfl(ox, gn - 1, 64);
+      const int xc_lo = ox_lo >> 2, xc_hi = min(3, (ox_hi + NS - 1) >> 2);
+      const int rowA = ((s >> 2) * SB + j) * TP + (s & 3) * SB + q;
+      mfma_f4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
+      for (int xc = xc_lo; xc <= xc_hi; ++xc) {
+        const float are = tre[rowA + 4 * xc], aim = tim[rowA + 4 * xc];
+        const int kk = 4 * xc + q - ox;
+        const float b = (valid && (unsigned)kk < (unsigned)NS) ? es_phi(tx + (float)kk, a.es.beta_log2e, a.es.c) : 0.0f;
+        dre = __builtin_amdgcn_mfma_f32_16x16x4f32(are, b, dre, 0, 0, 0);
+        dim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, b, dim, 0, 0, 0);
+      }
+      float pre = 0.0f, pim = 0.0f;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ky = 4 * q + e - oy;
+        const float w = (unsigned)ky < (unsigned)NS ? es_phi(ty + (float)ky, a.es.beta_log2e, a.es.c) : 0.0f;
+        pre = fmaf(w, dre[e], pre);
+        pim = fmaf(w, dim[e], pim);
+      }
+      pre += __shfl_xor(pre, 16, 64);
+      pim += __shfl_xor(pim, 16, 64);
+      pre += __shfl_xor(pre, 32, 64);
+      pim += __shfl_xor(pim, 32, 64);
+      if (q == 0 && valid) oT[rv.w] = cplx<float>{pre, pim};
+    }
+    __syncthreads();
+  }
+}
+
 // Whether a geometry matches the MFMA kernel's bin layout.
 inline bool mfma2d_applicable(const Geometry& g, int prec) {
+  // ns <= 8 keeps SB = 16 - ns >= 8, which the spread kernel's phased tile update needs
   if (g.dim != 2 || prec != kF32 || g.ns > 8) return false;
   const int bs = 4 * (16 - g.ns);
   return g.bs[0] == bs && g.bs[1] == bs;

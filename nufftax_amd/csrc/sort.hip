@@ -4,13 +4,18 @@
 // Contract: perm == argsort(key, stable) and bin_start == [0, cumsum(bincount(key))],
 // where key = sum_d floor(xs_d)/bs_d * prod_{e<d} nb_e and xs is the FP64 fold.
 //
-// Counting sort in three passes, all integer work:
-//   A  per-chunk histograms (LDS counters)        -> hist[bin][chunk]
-//   B  exclusive scan of hist in (bin, chunk) order -> first slot of every (bin, chunk)
-//   C  one wave per chunk walks its points in index order, ranks equal keys
-//      inside each 64-point step with ballots, and writes perm[slot] = index.
+// Counting sort, all integer work:
+//   A  k_hist: one block per chunk of points counts keys in LDS and writes its
+//      row of the histogram H[chunk][bin] (coalesced).
+//   B  segmented column scan of H (k_colsum_seg, k_bin_scan, k_row_offsets):
+//      H[chunk][bin] becomes the first output slot of (bin, chunk), bin_start
+//      the first slot of every bin.
+//   C  k_scatter: the W waves of a chunk block each count their quarter of the
+//      chunk, turn the counts into per-wave bases, then walk their points in
+//      index order, ranking equal keys inside each 64-point step with one ballot
+//      per key bit, and write perm[slot] = index.
 // A subproblem list (bins split into <= maxsub points) is then built for the
-// LDS-tiled spread/interp kernels.
+// tiled spread/interp kernels.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -56,27 +61,29 @@ __device__ __forceinline__ uint64_t match_key(int key, bool valid, int key_bits)
   return m;
 }
 
-constexpr int kHistThreads = 256;
-constexpr int kHistUnroll = 8;
+constexpr int kStep = 64;          // points per wave step
+constexpr int kGroup = 8;          // steps whose coordinates are loaded together
 
+// k_hist: block of 64*W threads counts the keys of chunk blockIdx.x ([lo, lo + CH)).
 template <typename CT, int D>
-__global__ void __launch_bounds__(kHistThreads) k_hist(KeyArgs ka, int64_t M, int CH, int NB, int nbins,
-                                                       int key_bits, int32_t* __restrict__ hist) {
+__global__ void __launch_bounds__(256) k_hist(KeyArgs ka, int64_t M, int CH, int nbins, int key_bits,
+                                              int32_t* __restrict__ H) {
   extern __shared__ int32_t cnt[];
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
   const int64_t lo = (int64_t)blockIdx.x * CH;
   const int64_t hi = min(M, lo + (int64_t)CH);
-  for (int64_t base = lo; base < hi; base += (int64_t)kHistThreads * kHistUnroll) {
-    double x[kHistUnroll][D];
+  const int stride = blockDim.x;
+  for (int64_t base = lo; base < hi; base += (int64_t)stride * kGroup) {
+    double x[kGroup][D];
 #pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      int64_t i = base + (int64_t)u * kHistThreads + threadIdx.x;
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + (int64_t)u * stride + threadIdx.x;
       if (i < hi) load_coords<CT, D>(ka, i, x[u]);
     }
 #pragma unroll
-    for (int u = 0; u < kHistUnroll; ++u) {
-      const int64_t i = base + (int64_t)u * kHistThreads + threadIdx.x;
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + (int64_t)u * stride + threadIdx.x;
       const bool valid = i < hi;
       const int key = valid ? key_of<D>(ka, x[u]) : 0;
       const uint64_t m = match_key(key, valid, key_bits);
@@ -84,36 +91,67 @@ __global__ void __launch_bounds__(kHistThreads) k_hist(KeyArgs ka, int64_t M, in
     }
   }
   __syncthreads();
-  for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[(int64_t)b * NB + blockIdx.x] = cnt[b];
+  int32_t* row = H + (int64_t)blockIdx.x * nbins;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) row[b] = cnt[b];
 }
 
-constexpr int kScatUnroll = 8;
-
+// k_scatter: W waves per chunk; wave w owns points [lo + w*SC, lo + (w+1)*SC).
 template <typename CT, int D>
-__global__ void __launch_bounds__(64) k_scatter(KeyArgs ka, int64_t M, int CH, int NB, int nbins, int key_bits,
-                                                const int32_t* __restrict__ offs, int32_t* __restrict__ perm) {
-  extern __shared__ int32_t base[];
-  const int lane = threadIdx.x;
-  for (int b = lane; b < nbins; b += 64) base[b] = offs[(int64_t)b * NB + blockIdx.x];
-  __syncthreads();
-  const int64_t lo = (int64_t)blockIdx.x * CH;
-  const int64_t hi = min(M, lo + (int64_t)CH);
-  for (int64_t grp = lo; grp < hi; grp += 64 * kScatUnroll) {
-    double x[kScatUnroll][D];
+__global__ void __launch_bounds__(256) k_scatter(KeyArgs ka, int64_t M, int SC, int nbins, int key_bits,
+                                                 const int32_t* __restrict__ H, int32_t* __restrict__ perm) {
+  extern __shared__ int32_t wcnt[];  // [W][nbins]
+  const int W = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int32_t* my = wcnt + wave * nbins;
+  for (int b = lane; b < nbins; b += 64) my[b] = 0;
+  const int64_t lo = (int64_t)blockIdx.x * W * SC + (int64_t)wave * SC;
+  const int64_t hi = min(M, lo + (int64_t)SC);
+  // count this wave's keys (leaders of equal keys update distinct counters)
+  for (int64_t base = lo; base < hi; base += kStep * kGroup) {
+    double x[kGroup][D];
 #pragma unroll
-    for (int u = 0; u < kScatUnroll; ++u) {
-      int64_t i = grp + u * 64 + lane;
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + u * kStep + lane;
       if (i < hi) load_coords<CT, D>(ka, i, x[u]);
     }
 #pragma unroll
-    for (int u = 0; u < kScatUnroll; ++u) {
-      const int64_t i = grp + u * 64 + lane;
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + u * kStep + lane;
+      const bool valid = i < hi;
+      const int key = valid ? key_of<D>(ka, x[u]) : 0;
+      const uint64_t m = match_key(key, valid, key_bits);
+      if (valid && popc_below(m) == 0) my[key] += (int)__popcll(m);
+    }
+  }
+  __syncthreads();
+  // per-wave bases: chunk slot of the bin, then earlier waves' counts
+  const int32_t* row = H + (int64_t)blockIdx.x * nbins;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+    int run = row[b];
+    for (int w = 0; w < W; ++w) {
+      const int t = wcnt[w * nbins + b];
+      wcnt[w * nbins + b] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  // rank in index order and write the permutation
+  for (int64_t base = lo; base < hi; base += kStep * kGroup) {
+    double x[kGroup][D];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + u * kStep + lane;
+      if (i < hi) load_coords<CT, D>(ka, i, x[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + u * kStep + lane;
       const bool valid = i < hi;
       const int key = valid ? key_of<D>(ka, x[u]) : 0;
       const uint64_t m = match_key(key, valid, key_bits);
       const int rank = popc_below(m);
-      const int b0 = base[key];
-      if (valid && rank == 0) base[key] = b0 + (int)__popcll(m);
+      const int b0 = my[key];
+      if (valid && rank == 0) my[key] = b0 + (int)__popcll(m);
       const int pos = b0 + rank;
       if (valid && pos >= 0 && (int64_t)pos < M) perm[pos] = (int32_t)i;
     }
@@ -121,10 +159,6 @@ __global__ void __launch_bounds__(64) k_scatter(KeyArgs ka, int64_t M, int CH, i
 }
 
 // ------------------------------------------------------------------ scan
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 16;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
 // Block-wide exclusive scan of one value per thread; returns the block total.
 template <int NT>
 __device__ __forceinline__ int block_excl_scan(int v, int& total) {
@@ -152,67 +186,72 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total) {
   return wbase + incl - v;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const int32_t* __restrict__ a, int64_t n,
-                                                              int32_t* __restrict__ part) {
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+constexpr int kRowSeg = 16;  // histogram rows per scan segment
+
+// P[seg][b] = sum of H[r][b] over the rows r of segment seg
+__global__ void __launch_bounds__(256) k_colsum_seg(const int32_t* __restrict__ H, int NB, int nbins,
+                                                    int32_t* __restrict__ P) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int seg = blockIdx.y;
+  if (b >= nbins) return;
+  const int r0 = seg * kRowSeg, r1 = min(NB, r0 + kRowSeg);
   int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    int64_t i = base + k;
-    if (i < n) s += a[i];
-  }
-  int total;
-  block_excl_scan<kScanThreads>(s, total);
-  if (threadIdx.x == 0) part[blockIdx.x] = total;
+  for (int r = r0; r < r1; ++r) s += H[(int64_t)r * nbins + b];
+  P[(int64_t)seg * nbins + b] = s;
 }
 
-// single block: exclusive scan of the partials in place
-__global__ void __launch_bounds__(1024) k_scan_partials(int32_t* __restrict__ part, int n) {
+// single block: P[seg][b] -> exclusive prefix over segments, bin totals -> exclusive
+// scan over bins (bin_start); P[seg][b] += bin_start[b].
+__global__ void __launch_bounds__(1024) k_bin_scan(int32_t* __restrict__ P, int nseg, int nbins, int M,
+                                                   int32_t* __restrict__ bin_start) {
   int carry = 0;
-  for (int base = 0; base < n; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < n ? part[i] : 0;
+  for (int base = 0; base < nbins; base += 1024) {
+    const int b = base + threadIdx.x;
+    int tot = 0;
+    if (b < nbins) {
+      for (int g = 0; g < nseg; ++g) {
+        const int t = P[(int64_t)g * nbins + b];
+        P[(int64_t)g * nbins + b] = tot;
+        tot += t;
+      }
+    }
     int total;
-    int ex = block_excl_scan<1024>(v, total);
-    if (i < n) part[i] = carry + ex;
+    const int ex = block_excl_scan<1024>(tot, total) + carry;
+    if (b < nbins) {
+      bin_start[b] = ex;
+      for (int g = 0; g < nseg; ++g) P[(int64_t)g * nbins + b] += ex;
+    }
     carry += total;
   }
+  if (threadIdx.x == 0) bin_start[nbins] = M;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_final(int32_t* __restrict__ a, int64_t n,
-                                                             const int32_t* __restrict__ part) {
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  int v[kScanItems];
-  int s = 0;
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    int64_t i = base + k;
-    v[k] = i < n ? a[i] : 0;
-    s += v[k];
-  }
-  int total;
-  int run = block_excl_scan<kScanThreads>(s, total) + part[blockIdx.x];
-#pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    int64_t i = base + k;
-    if (i < n) a[i] = run;
-    run += v[k];
+// H[r][b] -> first output slot of (bin b, chunk r)
+__global__ void __launch_bounds__(256) k_row_offsets(int32_t* __restrict__ H, int NB, int nbins,
+                                                     const int32_t* __restrict__ P) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int seg = blockIdx.y;
+  if (b >= nbins) return;
+  const int r0 = seg * kRowSeg, r1 = min(NB, r0 + kRowSeg);
+  int run = P[(int64_t)seg * nbins + b];
+  for (int r = r0; r < r1; ++r) {
+    const int t = H[(int64_t)r * nbins + b];
+    H[(int64_t)r * nbins + b] = run;
+    run += t;
   }
 }
 
-// single block: bin starts and subproblem descriptors {bin, first sorted slot, count}
-__global__ void __launch_bounds__(1024) k_subprob(const int32_t* __restrict__ offs, int NB, int nbins, int M,
-                                                  int maxsub, int max_subprob, int32_t* __restrict__ bin_start,
-                                                  int4* __restrict__ subprob, int32_t* __restrict__ nsub) {
+// single block: subproblem descriptors {bin, first sorted slot, count} from bin_start
+__global__ void __launch_bounds__(1024) k_subprob(const int32_t* __restrict__ bin_start, int nbins, int maxsub,
+                                                  int max_subprob, int4* __restrict__ subprob,
+                                                  int32_t* __restrict__ nsub) {
   int carry = 0;
   for (int base = 0; base < nbins; base += 1024) {
     const int b = base + threadIdx.x;
     int st = 0, cnt = 0;
     if (b < nbins) {
-      st = offs[(int64_t)b * NB];
-      const int en = (b + 1 < nbins) ? offs[(int64_t)(b + 1) * NB] : M;
-      cnt = en - st;
-      bin_start[b] = st;
+      st = bin_start[b];
+      cnt = bin_start[b + 1] - st;
     }
     const int nsb = (cnt + maxsub - 1) / maxsub;
     int total;
@@ -223,10 +262,7 @@ __global__ void __launch_bounds__(1024) k_subprob(const int32_t* __restrict__ of
     }
     carry += total;
   }
-  if (threadIdx.x == 0) {
-    bin_start[nbins] = M;
-    *nsub = min(carry, max_subprob);
-  }
+  if (threadIdx.x == 0) *nsub = min(carry, max_subprob);
 }
 
 __global__ void k_empty_sort(int nbins, int32_t* bin_start, int32_t* nsub) {
@@ -254,19 +290,19 @@ hipError_t grow(T*& p, int64_t& cap, int64_t need) {
 }
 
 template <typename CT, int D>
-void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, hipStream_t s) {
-  const size_t lds = (size_t)nbins * sizeof(int32_t);
+void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int SC, hipStream_t s) {
   int key_bits = 0;
   while ((1 << key_bits) < nbins) ++key_bits;
-  hipLaunchKernelGGL((k_hist<CT, D>), dim3(ps.NB), dim3(kHistThreads), lds, s, ka, ps.M, ps.CH, ps.NB, nbins,
+  const int CH = W * SC;
+  hipLaunchKernelGGL((k_hist<CT, D>), dim3(ps.NB), dim3(256), (size_t)nbins * sizeof(int32_t), s, ka, ps.M, CH, nbins,
                      key_bits, ps.hist);
-  const int64_t n = (int64_t)nbins * ps.NB;
-  const int nparts = (int)((n + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(kScanThreads), 0, s, ps.hist, n, ps.scan_part);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ps.scan_part, nparts);
-  hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kScanThreads), 0, s, ps.hist, n, ps.scan_part);
-  hipLaunchKernelGGL((k_scatter<CT, D>), dim3(ps.NB), dim3(64), lds, s, ka, ps.M, ps.CH, ps.NB, nbins, key_bits,
-                     ps.hist, ps.perm);
+  const int nseg = (ps.NB + kRowSeg - 1) / kRowSeg;
+  const dim3 g2((nbins + 255) / 256, nseg);
+  hipLaunchKernelGGL(k_colsum_seg, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, ps.scan_part, nseg, nbins, (int)ps.M, ps.bin_start);
+  hipLaunchKernelGGL(k_row_offsets, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part);
+  hipLaunchKernelGGL((k_scatter<CT, D>), dim3(ps.NB), dim3(64 * W), (size_t)W * nbins * sizeof(int32_t), s, ka, ps.M,
+                     SC, nbins, key_bits, ps.hist, ps.perm);
 }
 
 }  // namespace
@@ -284,22 +320,20 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   }
   ps.maxsub = tune.maxsub;
   const int64_t M = ps.M;
-  int64_t CH = tune.chunk;
-  int64_t NB = (M + CH - 1) / CH;
-  if (NB * nbins > tune.hist_cap) {
-    NB = std::max<int64_t>(1, tune.hist_cap / nbins);
-    CH = (M + NB - 1) / NB;
-    NB = (M + CH - 1) / CH;
-  }
-  if (NB < 1) NB = 1;
+  // waves per chunk: their LDS counters (W * nbins ints) must fit 64 KiB
+  const int W = nbins <= 4096 ? 4 : 1;
+  // points per wave: at least tune.chunk / 4, and enough that the histogram fits hist_cap
+  int64_t SC = std::max<int64_t>(tune.chunk / 4, 64);
+  const int64_t need = (M * (int64_t)nbins + tune.hist_cap - 1) / tune.hist_cap;  // min points per chunk
+  SC = std::max<int64_t>(SC, (need + W - 1) / W);
+  SC = (SC + 63) / 64 * 64;
+  if (SC > (1 << 24)) SC = 1 << 24;
+  const int64_t NB = std::max<int64_t>(1, (M + W * SC - 1) / (W * SC));
   ps.NB = (int)NB;
-  ps.CH = (int)std::max<int64_t>(CH, 1);
+  ps.CH = (int)(W * SC);
   const int64_t nhist = (int64_t)nbins * NB;
-  const int64_t nparts = (nhist + kScanTile - 1) / kScanTile;
-  if (nparts > 1024 * 1024) {
-    err = "histogram matrix too large";
-    return hipErrorInvalidValue;
-  }
+  const int64_t nseg = (NB + kRowSeg - 1) / kRowSeg;
+  const int64_t nparts = nseg * nbins;
   const int64_t max_sub = nbins + (M + ps.maxsub - 1) / ps.maxsub;
   if ((e = grow(ps.perm, ps.cap_M, M)) != hipSuccess) return e;
   if ((e = grow(ps.hist, ps.cap_hist, nhist)) != hipSuccess) return e;
@@ -320,17 +354,18 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
     ka.bs[d] = g.bs[d];
     ka.nb[d] = g.nb[d];
   }
+  const int isc = (int)SC;
   if (ps.coord_f64) {
-    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, s);
-    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, s);
-    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, s);
+    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, s);
+    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, s);
+    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, W, isc, s);
   } else {
-    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, s);
-    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, s);
-    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, s);
+    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, W, isc, s);
+    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, s);
+    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, s);
   }
-  hipLaunchKernelGGL(k_subprob, dim3(1), dim3(1024), 0, s, ps.hist, ps.NB, nbins, (int)M, ps.maxsub, ps.max_subprob,
-                     ps.bin_start, ps.subprob, ps.nsub);
+  hipLaunchKernelGGL(k_subprob, dim3(1), dim3(1024), 0, s, ps.bin_start, nbins, ps.maxsub, ps.max_subprob,
+                     ps.subprob, ps.nsub);
   return hipGetLastError();
 }
 

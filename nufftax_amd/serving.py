@@ -1,0 +1,394 @@
+"""Serving loop: many independent NUFFT requests on one GPU.
+
+There is no counterpart in the reference (its nearest relative is the n_trans
+batching of ``nufftax/core/spread.py:100-173`` and the batcher fast path of
+``nufftax/transforms/primitives.py:41-42``).  Design:
+
+* requests are type-1 or type-2 transforms (1D/2D/3D, complex64/complex128) whose
+  inputs already live on the GPU; ``submit`` returns a ``concurrent.futures.Future``;
+* a dispatcher thread admits requests against an HBM budget (points, strengths,
+  fine grid and output of every request in flight), groups requests with the
+  same plan signature that share their point set into one n_trans launch, and
+  issues each launch on one of ``n_streams`` HIP streams (round robin), each with
+  its own cached plans, so that small requests overlap on the GPU;
+* a completion thread waits on each launch's HIP event in issue order per
+  stream, resolves the futures and releases the budget;
+* every request records its arrival, start and completion time (host clock),
+  from which ``latency_stats`` reports p50 / p99; ``future.request`` is the
+  ``ServeRequest``.
+
+Results are those of the same plan called directly: the launch path is
+``nufftx_setpts`` + ``nufftx_execute`` on the request's stream.
+"""
+
+from __future__ import annotations
+
+import math
+import queue
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._plan import F32, F64, Plan
+from .params import compute_grid_size, compute_kernel_params
+
+__all__ = ["NufftServer", "ServeRequest", "latency_stats", "request_bytes"]
+
+
+@dataclass
+class ServeRequest:
+    kind: int                    # 1 or 2
+    coords: tuple                # (x, y, z)[:dim] device tensors, float32 / float64
+    data: torch.Tensor           # type 1: c (n_trans?, M); type 2: f (n_trans?, n_d, .., n_1)
+    n_modes: tuple               # (n1, .., nd)
+    eps: float
+    isign: int
+    future: Future = field(default_factory=Future)
+    ready: object = None         # HIP event recorded on the submitter's stream
+    stream_index: int | None = None  # pin to one of the server's streams (plan warm-up)
+    nbytes: int = 0
+    t_submit: float = 0.0
+    t_start: float = 0.0
+    t_done: float = 0.0
+
+    @property
+    def dim(self):
+        return len(self.coords)
+
+    def signature(self):
+        prec = F64 if (self.data.dtype == torch.complex128 or self.eps < 1e-7) else F32
+        return (self.kind, self.dim, tuple(self.n_modes), float(self.eps), int(self.isign), prec)
+
+
+def request_bytes(kind, dim, M, n_modes, eps, complex_bytes=8, n_trans=1):
+    """HBM held by one request in flight: points, strengths, output, plan fine grid and sort state."""
+    kp = compute_kernel_params(eps)
+    nf = math.prod(compute_grid_size(n, 2.0, kp.nspread) for n in n_modes)
+    nm = math.prod(n_modes)
+    real = complex_bytes // 2
+    return int(M * (dim * real + 4) + n_trans * (M + nm) * complex_bytes + n_trans * nf * complex_bytes)
+
+
+class NufftServer:
+    """Asynchronous NUFFT executor over ``n_streams`` HIP streams with HBM admission control."""
+
+    def __init__(self, device="cuda:0", n_streams: int = 4, hbm_budget_bytes: int = 32 << 30, max_group: int = 64):
+        if not torch.cuda.is_available():
+            raise _lib.NufftxError("NufftServer needs an AMD GPU")
+        self.device = torch.device(device)
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(n_streams)]  # default priority
+        self.budget = int(hbm_budget_bytes)
+        self.max_group = int(max_group)
+        self._in_use = 0
+        self._cv = threading.Condition()
+        self._q: "queue.Queue[ServeRequest | None]" = queue.Queue()
+        self._done_q: "queue.Queue" = queue.Queue()
+        self._next_stream = 0
+        self._plans = {}  # (signature, n_trans, stream) -> Plan, kept for the server's lifetime
+        self._closed = False
+        self._dispatcher = threading.Thread(target=self._dispatch_loop, name="nufftx-dispatch", daemon=True)
+        self._completer = threading.Thread(target=self._complete_loop, name="nufftx-complete", daemon=True)
+        self._dispatcher.start()
+        self._completer.start()
+
+    # ------------------------------------------------------------------ API
+    def submit(self, kind, coords, data, n_modes=None, eps=1e-6, isign=None, stream_index=None) -> Future:
+        """Queue a type-1 (kind=1, data = strengths) or type-2 (kind=2, data = modes) transform.
+        stream_index pins the request to one server stream (default: round robin)."""
+        if self._closed:
+            raise RuntimeError("server is closed")
+        coords = tuple(coords)
+        dim = len(coords)
+        if kind == 1:
+            if n_modes is None:
+                raise ValueError("type 1 needs n_modes")
+            n_modes = (int(n_modes),) * dim if isinstance(n_modes, int) else tuple(int(v) for v in n_modes)
+            isign = 1 if isign is None else isign
+        elif kind == 2:
+            n_modes = tuple(reversed(data.shape[data.ndim - dim:]))
+            isign = -1 if isign is None else isign
+        else:
+            raise ValueError("kind must be 1 or 2")
+        for t in coords + (data,):
+            if not (isinstance(t, torch.Tensor) and t.device == self.device):
+                raise ValueError("serving inputs must be tensors on the server's device")
+        M = coords[0].shape[-1]
+        lead = data.shape[:-1] if kind == 1 else data.shape[: data.ndim - dim]
+        n_trans = int(math.prod(lead)) if lead else 1
+        req = ServeRequest(kind, coords, data, n_modes, float(eps), 1 if isign > 0 else -1)
+        req.nbytes = request_bytes(kind, dim, M, n_modes, eps, data.element_size(), n_trans)
+        req.stream_index = None if stream_index is None else int(stream_index) % len(self.streams)
+        if req.nbytes > self.budget:
+            raise ValueError(f"request needs {req.nbytes} B of HBM, over the server budget of {self.budget} B")
+        req.ready = torch.cuda.Event()
+        req.ready.record(torch.cuda.current_stream(self.device))
+        req.future.request = req
+        req.t_submit = time.perf_counter()
+        self._q.put(req)
+        return req.future
+
+    def close(self):
+        if self._closed:
+            return
+        self._closed = True
+        self._q.put(None)
+        self._dispatcher.join()
+        self._done_q.put(None)
+        self._completer.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------ internals
+    def _admit(self, nbytes):
+        with self._cv:
+            while self._in_use + nbytes > self.budget:
+                self._cv.wait()
+            self._in_use += nbytes
+
+    def _release(self, nbytes):
+        with self._cv:
+            self._in_use -= nbytes
+            self._cv.notify_all()
+
+    def _take_group(self, first: ServeRequest):
+        """first plus queued requests with the same signature and the same point tensors."""
+        group = [first]
+        held = []
+        total = first.nbytes
+        while len(group) < self.max_group:
+            try:
+                r = self._q.get_nowait()
+            except queue.Empty:
+                break
+            if r is None:
+                held.append(r)
+                break
+            if r.signature() == first.signature() and all(a is b for a, b in zip(r.coords, first.coords)) and \
+                    r.stream_index == first.stream_index and \
+                    r.data.shape == first.data.shape and total + r.nbytes <= self.budget:
+                group.append(r)
+                total += r.nbytes
+            else:
+                held.append(r)
+        return group, held
+
+    def _dispatch_loop(self):
+        pending: list = []
+        while True:
+            r = pending.pop(0) if pending else self._q.get()
+            if r is None:
+                break
+            group, held = self._take_group(r)
+            pending.extend(held)
+            nbytes = sum(g.nbytes for g in group)
+            self._admit(nbytes)
+            if r.stream_index is not None:
+                stream = self.streams[r.stream_index]
+            else:
+                stream = self.streams[self._next_stream]
+                self._next_stream = (self._next_stream + 1) % len(self.streams)
+            t0 = time.perf_counter()
+            try:
+                with torch.cuda.stream(stream):
+                    for g in group:
+                        stream.wait_event(g.ready)
+                    outs = self._launch(group)
+                    ev = torch.cuda.Event()
+                    ev.record(stream)
+            except Exception as e:  # noqa: BLE001 - reported through the futures
+                self._release(nbytes)
+                for g in group:
+                    g.future.set_exception(e)
+                continue
+            for g in group:
+                g.t_start = t0
+            self._done_q.put((ev, group, outs, nbytes))
+
+    def _launch(self, group):
+        first = group[0]
+        kind, dim, n_modes, eps, isign, prec = first.signature()
+        cdt = torch.complex128 if prec == F64 else torch.complex64
+        rdt = torch.float64 if prec == F64 else torch.float32
+        coords = [x if (x.dtype == rdt and x.is_contiguous()) else x.to(rdt).contiguous() for x in first.coords]
+        M = coords[0].shape[0]
+        if kind == 1:
+            datas = [g.data.reshape(-1, M) for g in group]
+        else:
+            datas = [g.data.reshape((-1,) + tuple(reversed(n_modes))) for g in group]
+        sizes = [d.shape[0] for d in datas]
+        src = torch.cat(datas, 0).to(cdt).contiguous() if len(datas) > 1 else datas[0].to(cdt).contiguous()
+        n_trans = src.shape[0]
+        key = (first.signature(), n_trans, torch.cuda.current_stream(self.device).cuda_stream)
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = Plan.transform(kind, dim, n_modes, n_trans, eps, isign, 2.0, prec, 0, self.device)
+            self._plans[key] = plan
+        plan.setpts(coords)
+        if kind == 1:
+            out = torch.empty((n_trans,) + tuple(reversed(n_modes)), dtype=cdt, device=self.device)
+            plan.execute(src, out)
+        else:
+            out = torch.empty((n_trans, M), dtype=cdt, device=self.device)
+            plan.execute(out, src)
+        outs = []
+        off = 0
+        for g, n in zip(group, sizes):
+            o = out[off:off + n]
+            off += n
+            lead = g.data.shape[:-1] if kind == 1 else g.data.shape[: g.data.ndim - dim]
+            tail = tuple(reversed(n_modes)) if kind == 1 else (M,)
+            outs.append(o.reshape(tuple(lead) + tail).to(g.data.dtype))
+        return outs
+
+    def _complete_loop(self):
+        while True:
+            item = self._done_q.get()
+            if item is None:
+                break
+            ev, group, outs, nbytes = item
+            ev.synchronize()
+            t = time.perf_counter()
+            self._release(nbytes)
+            for g, o in zip(group, outs):
+                g.t_done = t
+                g.future.set_result(o)
+
+
+def latency_stats(requests) -> dict:
+    """p50 / p99 / mean latency (submit -> done) in ms, and requests per second over the span."""
+    lat = np.array([(r.t_done - r.t_submit) * 1e3 for r in requests])
+    span = max(r.t_done for r in requests) - min(r.t_submit for r in requests)
+    return {"n": len(lat), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "throughput_rps": len(lat) / span if span > 0 else float("inf")}
+
+
+# ---------------------------------------------------------------------- stream harness
+
+
+def mixed_requests(n: int, seed: int = 0, device="cuda:0"):
+    """The mixed serving workload: 70% 2D (n_modes 128^2 / 256^2 / 512^2, M log-uniform in
+    [1e4, 1e6]) and 30% 3D (32^3 / 64^3 / 128^3, M log-uniform in [1e5, 4e6]); type 1 or 2 with
+    probability 1/2; complex64, eps 1e-6.  Host draws (numpy default_rng(seed)), in order per
+    request: dimension, modes, log M, kind, then a torch generator seed for the device data."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        two_d = rng.random() < 0.7
+        if two_d:
+            n1 = int(rng.choice([128, 256, 512]))
+            nm, dim = (n1, n1), 2
+            M = int(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+        else:
+            n1 = int(rng.choice([32, 64, 128]))
+            nm, dim = (n1, n1, n1), 3
+            M = int(np.exp(rng.uniform(np.log(1e5), np.log(4e6))))
+        kind = 1 if rng.random() < 0.5 else 2
+        g = torch.Generator(device=device)
+        g.manual_seed(int(rng.integers(0, 2**31 - 1)))
+        coords = tuple((torch.rand(M, device=device, generator=g) * 2 - 1) * math.pi for _ in range(dim))
+        if kind == 1:
+            data = torch.randn(M, dtype=torch.complex64, device=device, generator=g)
+        else:
+            data = torch.randn(tuple(reversed(nm)), dtype=torch.complex64, device=device, generator=g)
+        out.append((kind, coords, data, nm))
+    return out
+
+
+def _direct_call(kind, coords, data, nm, eps=1e-6):
+    from . import transforms as T
+
+    dim = len(coords)
+    if kind == 1:
+        fn = [T.nufft1d1, T.nufft2d1, T.nufft3d1][dim - 1]
+        return fn(*coords, data, nm if dim > 1 else nm[0], eps=eps)
+    fn = [T.nufft1d2, T.nufft2d2, T.nufft3d2][dim - 1]
+    return fn(*coords, data, eps=eps)
+
+
+def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4):
+    """Replay reqs with Poisson arrivals at qps.  mode "server": NufftServer; mode "per-call":
+    the same transforms called one at a time (each synchronised) in arrival order."""
+    rng = np.random.default_rng(seed + 1)
+    gaps = rng.exponential(1.0 / qps, len(reqs))
+    arrivals = np.cumsum(gaps)
+    dev = reqs[0][1][0].device
+    torch.cuda.synchronize(dev)
+    records = []
+    if mode == "server":
+        with NufftServer(dev, n_streams=n_streams) as srv:
+            # build every stream's plans for the distinct shapes before the timed stream
+            firsts = {}
+            for r in reqs:
+                firsts.setdefault((r[0], len(r[1]), r[3]), r)
+            for si in range(n_streams):
+                warm = [srv.submit(k, c, d, n_modes=nm if k == 1 else None, eps=1e-6, stream_index=si)
+                        for k, c, d, nm in firsts.values()]
+                for f in warm:
+                    f.result()
+            torch.cuda.synchronize(dev)
+            t0 = time.perf_counter()
+            futs = []
+            for (kind, coords, data, nm), ta in zip(reqs, arrivals):
+                now = time.perf_counter() - t0
+                if ta > now:
+                    time.sleep(ta - now)
+                fut = srv.submit(kind, coords, data, n_modes=nm if kind == 1 else None, eps=1e-6)
+                fut.request.t_submit = t0 + ta  # latency counts from the scheduled arrival
+                futs.append(fut)
+            for f in futs:
+                f.result()
+            records = [f.request for f in futs]
+    else:
+        t0 = time.perf_counter()
+        for (kind, coords, data, nm), ta in zip(reqs, arrivals):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            r = ServeRequest(kind, coords, data, nm, 1e-6, 1 if kind == 1 else -1)
+            r.t_submit = t0 + ta
+            r.t_start = time.perf_counter()
+            _direct_call(kind, coords, data, nm)
+            torch.cuda.synchronize(dev)
+            r.t_done = time.perf_counter()
+            records.append(r)
+    return latency_stats(records)
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="p50/p99 latency of the mixed NUFFT serving stream")
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--qps", type=float, default=100.0)
+    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+    reqs = mixed_requests(args.requests, args.seed, dev)
+    # warm the plan caches of both paths with one pass over the distinct shapes
+    seen = set()
+    for kind, coords, data, nm in reqs:
+        key = (kind, len(coords), nm)
+        if key not in seen:
+            seen.add(key)
+            _direct_call(kind, coords, data, nm)
+    torch.cuda.synchronize(dev)
+    res = {"workload": "mixed 2D/3D type 1/2 stream, complex64, eps 1e-6", "requests": args.requests,
+           "qps": args.qps, "streams": args.streams,
+           "server": run_stream(reqs, args.qps, args.seed, "server", args.streams),
+           "per_call": run_stream(reqs, args.qps, args.seed, "per-call")}
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

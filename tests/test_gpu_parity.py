@@ -328,3 +328,26 @@ def test_spread2d_f32_every_nspread(rng, eps, dist):
     plan.spread(_t(c), fw)
     ref = R.spread([x.astype(np.float64) for x in coords], c.astype(np.complex128), nfs, kp)
     assert R.rel_l2(_np(fw), ref) < 2e-6
+
+
+@pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
+@pytest.mark.parametrize("dist", ["uniform", "clustered"])
+def test_interp2d_f32_every_nspread(rng, eps, dist):
+    """2D FP32 interpolation (matrix-core path for ns <= 8) vs the FP64 oracle, every even ns, batched."""
+    from nufftax_amd._plan import F32, grid_plan
+
+    kp = R.compute_kernel_params(eps)
+    nfs = (4 * (16 - kp.nspread) * 3 + 5, 4 * (16 - kp.nspread) * 2 + 3)
+    M = 20000
+    if dist == "uniform":
+        coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(2)]
+    else:
+        coords = [np.concatenate([rng.normal(0.3, 0.02, M - 500), rng.uniform(-np.pi, np.pi, 500)]) for _ in range(2)]
+    coords = [x.astype(np.float32) for x in coords]
+    fw = (rng.standard_normal((3, nfs[1], nfs[0])) + 1j * rng.standard_normal((3, nfs[1], nfs[0]))).astype(np.complex64)
+    plan = grid_plan(2, nfs, kp.nspread, kp.beta, kp.c, 3, F32, torch.device(DEV))
+    plan.setpts([_t(x) for x in coords])
+    c = torch.empty((3, M), dtype=torch.complex64, device=DEV)
+    plan.interp(_t(fw), c)
+    ref = R.interp([x.astype(np.float64) for x in coords], fw.astype(np.complex128), kp)
+    assert R.rel_l2(_np(c), ref) < 2e-6

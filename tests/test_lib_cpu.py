@@ -125,3 +125,79 @@ def test_workload_generators_are_seeded():
         np.testing.assert_array_equal(u, v)
     x, y, c, f = a
     assert x.shape == (80,) and f.shape == (4, 4) and np.all(np.abs(x) <= np.pi) and np.all(np.abs(y) <= np.pi)
+
+
+@pytest.mark.parametrize("nf,eps", [(64, 1e-6), (270, 1e-9), (1000, 1e-4), (2048, 1e-12), (7, 1e-2)])
+def test_kernel_fourier_series_matches_oracle(nf, eps):
+    from nufftax_amd import core
+
+    kp = R.compute_kernel_params(eps)
+    got = core.kernel_fourier_series(nf, kp.nspread, kp.beta, kp.c, dtype=torch_f64())
+    ref = R.kernel_fourier_series(nf, kp.nspread, kp.beta, kp.c)
+    np.testing.assert_allclose(got.numpy(), ref, rtol=1e-13, atol=1e-15)
+
+
+def torch_f64():
+    import torch
+
+    return torch.float64
+
+
+def test_core_kernel_helpers_match_oracle(rng):
+    import torch
+
+    from nufftax_amd import core
+
+    kp = R.compute_kernel_params(1e-6)
+    z = np.concatenate([rng.uniform(-5, 5, 200), [0.0, kp.nspread / 2, -kp.nspread / 2, 4.0000001]])
+    for dt in (np.float32, np.float64):
+        zz = z.astype(dt)
+        phi = core.es_kernel(torch.from_numpy(zz), kp.beta, kp.c).numpy()
+        np.testing.assert_allclose(phi, R.es_kernel(zz, kp.beta, kp.c), rtol=2e-6 if dt == np.float32 else 1e-14,
+                                   atol=1e-30)
+        p2, d2 = core.es_kernel_with_derivative(torch.from_numpy(zz), kp.beta, kp.c)
+        rp, rd = R.es_kernel_with_derivative(zz, kp.beta, kp.c)
+        tol = 2e-5 if dt == np.float32 else 1e-13
+        np.testing.assert_allclose(p2.numpy(), rp, rtol=tol, atol=1e-30)
+        np.testing.assert_allclose(d2.numpy(), rd, rtol=tol, atol=1e-25)
+    x = rng.uniform(-10, 10, 1000)
+    np.testing.assert_allclose(core.fold_rescale(torch.from_numpy(x), 512).numpy(), R.fold_rescale(x, 512),
+                               rtol=1e-12, atol=1e-9)
+
+
+def test_core_rejects_custom_kernels_and_needs_gpu():
+    import torch
+
+    from nufftax_amd import core
+
+    kp = core.compute_kernel_params(1e-6)
+    custom = core.Kernel(nspread=8, phi=lambda z: torch.exp(-z * z))
+    with pytest.raises(NotImplementedError):
+        core._es_params(custom)
+    assert core._es_params(core.es_kernel_spec(kp)) == kp
+    if not torch.cuda.is_available():
+        with pytest.raises(Exception):
+            core.spread_1d(np.zeros(4), np.ones(4, complex), 16, kp)
+
+
+def test_serving_host_logic():
+    import torch
+
+    from nufftax_amd import serving as S
+
+    b1 = S.request_bytes(1, 2, 10**5, (128, 128), 1e-6)
+    b2 = S.request_bytes(1, 2, 10**6, (128, 128), 1e-6)
+    b3 = S.request_bytes(1, 2, 10**6, (512, 512), 1e-6)
+    assert 0 < b1 < b2 < b3
+    # 1e6 points: x, y (4 B) + 4 B sort index + 8 B strengths; 512^2 modes and the 1024^2 fine grid
+    assert b3 == 10**6 * 12 + (10**6 + 512 * 512) * 8 + 1024 * 1024 * 8
+    recs = []
+    for i in range(100):
+        r = S.ServeRequest(1, (), None, (8,), 1e-6, 1)
+        r.t_submit, r.t_done = float(i), float(i) + 0.001 * (i + 1)
+        recs.append(r)
+    st = S.latency_stats(recs)
+    assert st["n"] == 100 and abs(st["p50_ms"] - 50.5) < 1 and st["p99_ms"] > 98
+    if not torch.cuda.is_available():
+        with pytest.raises(Exception):
+            S.NufftServer()
