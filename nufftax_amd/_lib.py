@@ -75,8 +75,8 @@ class NufftxError(RuntimeError):
 
 
 def build(jobs: int = 8, verbose: bool = False) -> str:
-    """Compile libnufftx.so for gfx950 with hipcc (make in csrc/)."""
-    cmd = ["make", "-C", CSRC, f"-j{jobs}"]
+    """Compile libnufftx.so for gfx950 with hipcc (make -B in csrc/: every source, every time)."""
+    cmd = ["make", "-B", "-C", CSRC, f"-j{jobs}"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise NufftxError(f"building libnufftx.so failed:\n{res.stdout[-4000:]}\n{res.stderr[-4000:]}")

@@ -158,10 +158,9 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
     wave_lds_fence();  // this chunk's record reads before the next chunk's writes
   }
   // ---- accumulators -> this wave's tile.  Windows of neighbouring sub-bins overlap,
-  // so lanes of one wave would update the same cells; the windows are added in 4
-  // phases by (sx, sy) parity -- within a phase they are disjoint (2 SB >= 16) --
-  // with a wavefront fence between phases so the compiler cannot move one phase's
-  // LDS reads above the previous phase's writes (LDS ops of a wave run in order).
+  // so lanes of one wave update the same cells; the windows are added in 4 phases
+  // by (sx, sy) parity -- within a phase they are disjoint (2 SB >= 16) -- with a
+  // block barrier between phases (every wave runs the same 4 phases).
   const int col = lane & 15, row0 = 4 * (lane >> 4);
 #pragma unroll
   for (int ph = 0; ph < 4; ++ph) {
@@ -176,9 +175,8 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
         tim[y * T + x] += acc_im[s][r];
       }
     }
-    wave_lds_fence();
+    __syncthreads();
   }
-  __syncthreads();
   // ---- sum the wave tiles, flush to the fine grid
   cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
   const int tile3[3] = {T, T, 1};
