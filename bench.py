@@ -192,7 +192,8 @@ def main():
     nfc = nf[0] * nf[1]
     bytes_spread = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c gathered; grid written once
     bytes_interp = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c written; grid read once
-    mfma = os.environ.get("NUFFTX_NO_MFMA") is None  # the 2D FP32 ns <= 8 path runs on the matrix cores
+    # the 2D FP32 ns <= 8 path runs on the matrix cores when NUFFTX_MFMA is set
+    mfma = os.environ.get("NUFFTX_MFMA") is not None and os.environ.get("NUFFTX_NO_MFMA") is None
     if t_spread >= t_interp:
         kname = "k_spread_mfma2d<float,8>" if mfma else "k_spread_tiled<float,float,2,8>"
         kt, kb = t_spread, bytes_spread

@@ -23,6 +23,8 @@
 //   D[i][j] in lane j + 16 (i / 4), element i % 4.
 #pragma once
 
+#include <cstdlib>
+
 #include "si_kernels.hpp"
 
 namespace nufftx {
@@ -203,10 +205,11 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
 // rows of D with the y-weights and 2 cross-lane adds finish the sum over y.
 //
 // A block takes one subproblem: the bin's tile (bin + halo) is loaded into LDS
-// once; points are folded (FP64) and counting-sorted in LDS by (sub-bin, x offset)
-// in segments of kInterpSeg, so every group of 16 points shares a window and a
-// narrow range of x offsets: column steps outside the group's support are skipped.
-constexpr int kInterpSeg = 1024;
+// once.  Points are taken in segments of kInterpSeg: each is folded (FP64), its
+// 2 ns weights are evaluated once, and the records are counting-sorted in LDS by
+// (sub-bin, x offset), so every group of 16 points shares a window and a narrow
+// range of x offsets: column steps outside the group's support are skipped.
+constexpr int kInterpSeg = 512;
 constexpr int kInterpThreads = 256;
 
 template <int NS>
@@ -217,14 +220,17 @@ struct InterpGeom {
   static constexpr int TCP = T * TP;
   static constexpr int NKEY = 16 * (SB + 1);
   static constexpr int MAXG = kInterpSeg / 16 + 16;
-  static constexpr size_t lds_bytes() { return 2 * (size_t)TCP * sizeof(float) + (size_t)kInterpSeg * sizeof(int4); }
+  static constexpr int REC = 2 * NS + 2;    // wx[NS], wy[NS], offsets | sub-bin, point index
+  static constexpr size_t lds_bytes() {
+    return 2 * (size_t)TCP * sizeof(float) + (size_t)kInterpSeg * REC * sizeof(float);
+  }
 };
 
 template <typename CT, int NS>
 __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> a, const cplx<float>* __restrict__ fw,
                                                                    cplx<float>* __restrict__ out) {
   using G = InterpGeom<NS>;
-  constexpr int SB = G::SB, T = G::T, TP = G::TP, TCP = G::TCP, NKEY = G::NKEY;
+  constexpr int SB = G::SB, T = G::T, TP = G::TP, TCP = G::TCP, NKEY = G::NKEY, REC = G::REC;
   constexpr int PPT = kInterpSeg / kInterpThreads;
   extern __shared__ __align__(16) float smem_f[];
   __shared__ int cnt[NKEY + 1];
@@ -232,7 +238,7 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
   __shared__ int ngrp;
   float* tre = smem_f;
   float* tim = tre + TCP;
-  int4* rec = reinterpret_cast<int4*>(tim + TCP);
+  float* rec = tim + TCP;
 
   const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
@@ -259,9 +265,9 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
   for (int seg0 = 0; seg0 < sd.z; seg0 += kInterpSeg) {
     for (int k = tid; k <= NKEY; k += kInterpThreads) cnt[k] = 0;
     __syncthreads();
-    // ---- fold, sub-bin, key; per-key slot by LDS atomics
-    int4 r[PPT];
-    int key[PPT], slot[PPT];
+    // ---- fold, weights, sub-bin, key; per-key slot by LDS atomics
+    float w[PPT][2 * NS];
+    int meta[PPT], pidx[PPT], key[PPT], slot[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
       const int jj = seg0 + u * kInterpThreads + tid;
@@ -275,8 +281,14 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
           const int sx = (fx.cell - bin_x0) / SB, sy = (fy.cell - bin_y0) / SB;
           const int ox = (i0x - org[0]) - sx * SB, oy = (i0y - org[1]) - sy * SB;
           const int s = sy * 4 + sx;
-          r[u] = make_int4(__float_as_int((float)((double)i0x - fx.xs)), __float_as_int((float)((double)i0y - fy.xs)),
-                           ox | (oy << 8) | (s << 16), p);
+          const float tx = (float)((double)i0x - fx.xs), ty = (float)((double)i0y - fy.xs);
+#pragma unroll
+          for (int k = 0; k < NS; ++k) {
+            w[u][k] = es_phi(tx + (float)k, a.es.beta_log2e, a.es.c);
+            w[u][NS + k] = es_phi(ty + (float)k, a.es.beta_log2e, a.es.c);
+          }
+          meta[u] = ox | (oy << 8) | (s << 16);
+          pidx[u] = p;
           key[u] = s * (SB + 1) + ox;
           slot[u] = atomicAdd(&cnt[key[u]], 1);
         }
@@ -330,8 +342,14 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
     }
     // ---- scatter records in key order
 #pragma unroll
-    for (int u = 0; u < PPT; ++u)
-      if (key[u] >= 0) rec[cnt[key[u]] + slot[u]] = r[u];
+    for (int u = 0; u < PPT; ++u) {
+      if (key[u] < 0) continue;
+      float* r = rec + (cnt[key[u]] + slot[u]) * REC;
+#pragma unroll
+      for (int k = 0; k < 2 * NS; ++k) r[k] = w[u][k];
+      r[2 * NS] = __int_as_float(meta[u]);
+      r[2 * NS + 1] = __int_as_float(pidx[u]);
+    }
     __syncthreads();
     // ---- groups of 16 points: D = T_win Wx on the matrix cores, then the y sum
     const int ng_total = ngrp;
@@ -339,9 +357,9 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
       const int2 gd = grp[g];
       const int s = gd.y & 255, gn = gd.y >> 8;
       const bool valid = j < gn;
-      const int4 rv = rec[gd.x + min(j, gn - 1)];
-      const float tx = __int_as_float(rv.x), ty = __int_as_float(rv.y);
-      const int ox = rv.z & 255, oy = (rv.z >> 8) & 255;
+      const float* r = rec + (gd.x + min(j, gn - 1)) * REC;
+      const int m = __float_as_int(r[2 * NS]);
+      const int ox = m & 255, oy = (m >> 8) & 255;
       const int ox_lo = __shfl(ox, 0, 64), ox_hi = __shfl(ox, gn - 1, 64);
       const int xc_lo = ox_lo >> 2, xc_hi = min(3, (ox_hi + NS - 1) >> 2);
       const int rowA = ((s >> 2) * SB + j) * TP + (s & 3) * SB + q;
@@ -349,7 +367,7 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
       for (int xc = xc_lo; xc <= xc_hi; ++xc) {
         const float are = tre[rowA + 4 * xc], aim = tim[rowA + 4 * xc];
         const int kk = 4 * xc + q - ox;
-        const float b = (valid && (unsigned)kk < (unsigned)NS) ? es_phi(tx + (float)kk, a.es.beta_log2e, a.es.c) : 0.0f;
+        const float b = (valid && (unsigned)kk < (unsigned)NS) ? r[kk] : 0.0f;
         dre = __builtin_amdgcn_mfma_f32_16x16x4f32(are, b, dre, 0, 0, 0);
         dim = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, b, dim, 0, 0, 0);
       }
@@ -357,18 +375,24 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int ky = 4 * q + e - oy;
-        const float w = (unsigned)ky < (unsigned)NS ? es_phi(ty + (float)ky, a.es.beta_log2e, a.es.c) : 0.0f;
-        pre = fmaf(w, dre[e], pre);
-        pim = fmaf(w, dim[e], pim);
+        const float wy = (unsigned)ky < (unsigned)NS ? r[NS + ky] : 0.0f;
+        pre = fmaf(wy, dre[e], pre);
+        pim = fmaf(wy, dim[e], pim);
       }
       pre += __shfl_xor(pre, 16, 64);
       pim += __shfl_xor(pim, 16, 64);
       pre += __shfl_xor(pre, 32, 64);
       pim += __shfl_xor(pim, 32, 64);
-      if (q == 0 && valid) oT[rv.w] = cplx<float>{pre, pim};
+      if (q == 0 && valid) oT[__float_as_int(r[2 * NS + 1])] = cplx<float>{pre, pim};
     }
     __syncthreads();
   }
+}
+
+// The matrix-core kernels are opt-in (NUFFTX_MFMA=1) until they beat the LDS-tiled
+// kernels on the 2D benchmark; NUFFTX_NO_MFMA=1 forces them off.
+inline bool use_mfma() {
+  return std::getenv("NUFFTX_MFMA") != nullptr && std::getenv("NUFFTX_NO_MFMA") == nullptr;
 }
 
 // Whether a geometry matches the MFMA kernel's bin layout.

@@ -10,7 +10,7 @@
 //   B  segmented column scan of H (k_colsum_seg, k_bin_scan, k_row_offsets):
 //      H[chunk][bin] becomes the first output slot of (bin, chunk), bin_start
 //      the first slot of every bin.
-//   C  k_scatter: the W waves of a chunk block each count their quarter of the
+//   C  k_scatter: the W waves of a chunk block each count their share of the
 //      chunk, turn the counts into per-wave bases, then walk their points in
 //      index order, ranking equal keys inside each 64-point step with one ballot
 //      per key bit, and write perm[slot] = index.
@@ -66,7 +66,7 @@ constexpr int kGroup = 8;          // steps whose coordinates are loaded togethe
 
 // k_hist: block of 64*W threads counts the keys of chunk blockIdx.x ([lo, lo + CH)).
 template <typename CT, int D>
-__global__ void __launch_bounds__(256) k_hist(KeyArgs ka, int64_t M, int CH, int nbins, int key_bits,
+__global__ void __launch_bounds__(1024) k_hist(KeyArgs ka, int64_t M, int CH, int nbins, int key_bits,
                                               int32_t* __restrict__ H) {
   extern __shared__ int32_t cnt[];
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_hist(KeyArgs ka, int64_t M, int CH, int
 
 // k_scatter: W waves per chunk; wave w owns points [lo + w*SC, lo + (w+1)*SC).
 template <typename CT, int D>
-__global__ void __launch_bounds__(256) k_scatter(KeyArgs ka, int64_t M, int SC, int nbins, int key_bits,
+__global__ void __launch_bounds__(1024) k_scatter(KeyArgs ka, int64_t M, int SC, int nbins, int key_bits,
                                                  const int32_t* __restrict__ H, int32_t* __restrict__ perm) {
   extern __shared__ int32_t wcnt[];  // [W][nbins]
   const int W = blockDim.x >> 6;
@@ -294,8 +294,8 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
   int key_bits = 0;
   while ((1 << key_bits) < nbins) ++key_bits;
   const int CH = W * SC;
-  hipLaunchKernelGGL((k_hist<CT, D>), dim3(ps.NB), dim3(256), (size_t)nbins * sizeof(int32_t), s, ka, ps.M, CH, nbins,
-                     key_bits, ps.hist);
+  hipLaunchKernelGGL((k_hist<CT, D>), dim3(ps.NB), dim3(64 * W), (size_t)nbins * sizeof(int32_t), s, ka, ps.M, CH,
+                     nbins, key_bits, ps.hist);
   const int nseg = (ps.NB + kRowSeg - 1) / kRowSeg;
   const dim3 g2((nbins + 255) / 256, nseg);
   hipLaunchKernelGGL(k_colsum_seg, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part);
@@ -320,10 +320,11 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   }
   ps.maxsub = tune.maxsub;
   const int64_t M = ps.M;
-  // waves per chunk: their LDS counters (W * nbins ints) must fit 64 KiB
-  const int W = nbins <= 4096 ? 4 : 1;
-  // points per wave: at least tune.chunk / 4, and enough that the histogram fits hist_cap
-  int64_t SC = std::max<int64_t>(tune.chunk / 4, 64);
+  // waves per chunk (up to 16): their LDS counters (W * nbins ints) must fit 64 KiB
+  int W = 16;
+  while (W > 1 && (int64_t)W * nbins * (int64_t)sizeof(int32_t) > 65536) W /= 2;
+  // points per wave: chunk / W, and enough that the histogram fits hist_cap
+  int64_t SC = std::max<int64_t>(tune.chunk / W, 64);
   const int64_t need = (M * (int64_t)nbins + tune.hist_cap - 1) / tune.hist_cap;  // min points per chunk
   SC = std::max<int64_t>(SC, (need + W - 1) / W);
   SC = (SC + 63) / 64 * 64;

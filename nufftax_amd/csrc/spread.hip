@@ -54,7 +54,7 @@ hipError_t spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_
 template <typename T, typename CT>
 hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
-    if (mfma2d_applicable(*ga.g, ga.prec) && std::getenv("NUFFTX_NO_MFMA") == nullptr)
+    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma())
       return spread_mfma<CT>(ga, c, fw, s);
   }
   switch (ga.g->dim) {

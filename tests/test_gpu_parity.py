@@ -309,10 +309,13 @@ def test_type3_vs_direct(rng, dim, prec):
 
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
-def test_spread2d_f32_every_nspread(rng, eps, dist):
-    """2D FP32 spreading (matrix-core path for ns <= 8) vs the FP64 oracle, every even ns."""
+@pytest.mark.parametrize("mfma", ["1", None])
+def test_spread2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
+    """2D FP32 spreading (LDS-tiled, and the matrix-core kernel for ns <= 8) vs the FP64 oracle, every even ns."""
     from nufftax_amd._plan import F32, grid_plan
 
+    if mfma:
+        monkeypatch.setenv("NUFFTX_MFMA", mfma)
     kp = R.compute_kernel_params(eps)
     nfs = (4 * (16 - kp.nspread) * 3 + 5, 4 * (16 - kp.nspread) * 2 + 3)  # several bins, ragged edge
     M = 20000
@@ -332,10 +335,14 @@ def test_spread2d_f32_every_nspread(rng, eps, dist):
 
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
-def test_interp2d_f32_every_nspread(rng, eps, dist):
-    """2D FP32 interpolation (matrix-core path for ns <= 8) vs the FP64 oracle, every even ns, batched."""
+@pytest.mark.parametrize("mfma", ["1", None])
+def test_interp2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
+    """2D FP32 interpolation (LDS-tiled, and the matrix-core kernel for ns <= 8) vs the FP64 oracle, every even
+    ns, batched."""
     from nufftax_amd._plan import F32, grid_plan
 
+    if mfma:
+        monkeypatch.setenv("NUFFTX_MFMA", mfma)
     kp = R.compute_kernel_params(eps)
     nfs = (4 * (16 - kp.nspread) * 3 + 5, 4 * (16 - kp.nspread) * 2 + 3)
     M = 20000
