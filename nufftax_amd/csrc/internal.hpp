@@ -38,12 +38,14 @@ struct PointSet {
   int max_subprob = 0;
   // device buffers (owned)
   int32_t* perm = nullptr;       // M
+  int32_t* keys = nullptr;       // M, bin key of every point (index order)
   int32_t* hist = nullptr;       // nbins * NB (+ scan scratch)
   int32_t* scan_part = nullptr;  // scan partials
   int32_t* bin_start = nullptr;  // nbins + 1
+  int32_t* sub_start = nullptr;  // nbins + 1, first subproblem of every bin
   int4* subprob = nullptr;       // max_subprob
   int32_t* nsub = nullptr;       // 1
-  int64_t cap_M = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0;
+  int64_t cap_M = 0, cap_keys = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0, cap_sub_start = 0;
 };
 
 struct SortTuning {

@@ -56,7 +56,7 @@ hipError_t interp_mfma(const GridOpArgs& ga, const void* fw, void* c, hipStream_
 template <typename T, typename CT>
 hipError_t interp_dim(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
-    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma())
+    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma_interp())
       return interp_mfma<CT>(ga, fw, c, s);
   }
   switch (ga.g->dim) {

@@ -1,4 +1,4 @@
-// 2D FP32 spreading on the matrix cores (gfx950 MFMA).
+// 2D FP32 spreading and interpolation on the matrix cores (gfx950 MFMA).
 //
 // The contribution of points j to a tile is a sum of outer products,
 //   tile[y][x] += sum_j (c_j wy_j[y - y0_j]) * wx_j[x - x0_j]      (nufftax/core/spread.py:465-489)
@@ -7,16 +7,17 @@
 // 4 x 4 sub-bins of SB = 16 - ns cells is spread with 16x16 windows: every point
 // of a sub-bin has its ns x ns support inside the sub-bin's 16 x 16 window, so one
 // v_mfma_f32_16x16x4f32 accumulates 4 points into the window (4 x 16 A operand =
-// c * y-weights, 16 x 4 B operand = x-weights).  Accumulators of all 16 sub-bins
-// live in registers for the whole subproblem; LDS only stages per-point weights
-// and the final tile.  Nothing is accumulated with LDS atomics.
+// c * y-weights, 16 x 4 B operand = x-weights).
 //
-// Work split: a block of kWaves waves takes one subproblem (<= maxsub points of
-// one bin); wave w processes 64-point chunks w, w + kWaves, ...  Per chunk every
-// lane folds one point (FP64), evaluates its 2 ns weights once, and the wave
-// orders the chunk by sub-bin with 16 ballots.  Each wave then adds its
-// accumulators into its own LDS tile; the block sums the wave tiles and flushes
-// non-zero cells to the fine grid with global atomics.
+// Spreading: a block of 4 waves takes one subproblem (<= maxsub points of one
+// bin) in segments of kSpreadSeg points.  Each thread folds one point (FP64),
+// evaluates its 2 ns weights once and the block counting-sorts the segment by
+// sub-bin into LDS records.  Wave w owns one sub-bin window of each of the 4
+// parity classes (sx & 1, sy & 1) and keeps their accumulators in registers for
+// the whole subproblem; at the end the 4 classes are added into the LDS tile one
+// after another (windows of a class are disjoint), and non-zero tile cells are
+// flushed to the fine grid with global atomics.  Nothing is accumulated with LDS
+// atomics.
 //
 // MFMA operand layout (v_mfma_f32_16x16x4f32, 64 lanes):
 //   A[i][k] in lane i + 16 k,  B[k][j] in lane j + 16 k,
@@ -31,14 +32,12 @@ namespace nufftx {
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// Orders LDS accesses of different lanes of one wave: the hardware runs a wave's
-// LDS instructions in order, the fence keeps the compiler from reordering them.
-__device__ __forceinline__ void wave_lds_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// A value the caller knows to be the same in every lane of the wave, moved to a
+// scalar register so that loops bounded by it are scalar (non-divergent) loops.
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-constexpr int kMfmaWaves = 2;
+constexpr int kSpreadSeg = 256;      // points staged per segment (one per thread)
+constexpr int kSpreadThreads = 256;  // 4 waves
 
 template <int NS>
 struct MfmaGeom {
@@ -48,55 +47,64 @@ struct MfmaGeom {
   static constexpr int T = BS + NS;        // tile side
   static constexpr int TC = T * T;
   static constexpr int REC = 3 * NS + 1;   // staged floats per point: wx[NS], (c wy)[NS] complex, offsets
-  static constexpr int WAVE_FLOATS = 2 * TC + 64 * REC;
-  static constexpr size_t lds_bytes() { return (size_t)kMfmaWaves * WAVE_FLOATS * sizeof(float); }
+  static constexpr size_t lds_bytes() {
+    return (2 * (size_t)TC + (size_t)kSpreadSeg * REC) * sizeof(float);
+  }
 };
 
+// Sub-bin s = sy * 4 + sx belongs to parity class (sx & 1) | (sy & 1) << 1; the 4
+// windows of a class are disjoint (2 SB >= 16).  Wave w owns the w-th sub-bin of
+// every class.
+__device__ __forceinline__ int owned_subbin(int wave, int cls) {
+  const int sx = (cls & 1) + 2 * (wave & 1), sy = (cls >> 1) + 2 * (wave >> 1);
+  return sy * 4 + sx;
+}
+
 template <typename CT, int NS>
-__global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float> a, const cplx<float>* __restrict__ c,
-                                                                    cplx<float>* __restrict__ fw) {
+__global__ void __launch_bounds__(kSpreadThreads) k_spread_mfma2d(SIArgs<float> a, const cplx<float>* __restrict__ c,
+                                                                   cplx<float>* __restrict__ fw) {
   using G = MfmaGeom<NS>;
   constexpr int SB = G::SB, T = G::T, TC = G::TC, REC = G::REC;
   extern __shared__ __align__(16) float smem_f[];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  float* tre = smem_f + wave * G::WAVE_FLOATS;
+  __shared__ int cnt[17];
+  float* tre = smem_f;
   float* tim = tre + TC;
-  float* stage = tim + TC;
+  float* rec = tim + TC;
 
   const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
   const int tr = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = uniform(tid >> 6);
   const int4 sd = a.subprob[sub];
   int org[3];
   tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
-  const int bin_x0 = org[0] + NS / 2, bin_y0 = org[1] + NS / 2;  // first cell of the bin
-  for (int i = lane; i < TC; i += 64) {
+  const int bin_x0 = org[0] + NS / 2, bin_y0 = org[1] + NS / 2;
+  for (int i = tid; i < TC; i += kSpreadThreads) {
     tre[i] = 0.0f;
     tim[i] = 0.0f;
   }
   const cplx<float>* cT = c + (int64_t)tr * a.M;
 
-  mfma_f4 acc_re[16], acc_im[16];
+  mfma_f4 acc_re[4], acc_im[4];  // this wave's window of each parity class
 #pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    acc_re[s] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-    acc_im[s] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < 4; ++k) {
+    acc_re[k] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    acc_im[k] = mfma_f4{0.f, 0.f, 0.f, 0.f};
   }
   const int kk = lane >> 4;  // point of the 4-point group this lane feeds
   const int rr = lane & 15;  // window row (A) / column (B) this lane feeds
 
-  const int nchunk = (sd.z + 63) / 64;
-  for (int ch = wave; ch < nchunk; ch += kMfmaWaves) {
-    // ---- per-point fold, weights and sub-bin (one point per lane)
-    const int j = ch * 64 + lane;
-    bool valid = j < sd.z;
-    int sbin = 16, offs = 0;
+  for (int seg0 = 0; seg0 < sd.z; seg0 += kSpreadSeg) {
+    if (tid <= 16) cnt[tid] = 0;
+    __syncthreads();
+    // ---- one point per thread: fold (FP64), weights, sub-bin, slot
+    const int j = seg0 + tid;
+    int s = -1, slot = 0, offs = 0;
     float wx[NS], cwr[NS], cwi[NS];
-    if (valid) {
+    if (j < sd.z) {
       const int p = a.perm[sd.y + j];
-      valid = (unsigned)p < (unsigned)a.M;
-      if (valid) {
+      if ((unsigned)p < (unsigned)a.M) {
         const cplx<float> cv = cT[p];
         const Folded fx = fold_coord((double)static_cast<const CT*>(a.coord[0])[p], a.nf[0]);
         const Folded fy = fold_coord((double)static_cast<const CT*>(a.coord[1])[p], a.nf[1]);
@@ -104,7 +112,7 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
         const float tx = (float)((double)i0x - fx.xs), ty = (float)((double)i0y - fy.xs);
         const int sx = (fx.cell - bin_x0) / SB, sy = (fy.cell - bin_y0) / SB;
         const int ox = (i0x - org[0]) - sx * SB, oy = (i0y - org[1]) - sy * SB;  // in [0, SB]
-        sbin = sy * 4 + sx;
+        s = sy * 4 + sx;
         offs = ox | (oy << 8);
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
@@ -113,21 +121,26 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
           cwr[k] = cv.re * wy;
           cwi[k] = cv.im * wy;
         }
+        slot = atomicAdd(&cnt[s], 1);
       }
     }
-    // ---- order the chunk by sub-bin (16 ballots), stage records
-    int cnt_s[16], start_s[16];
-    int pos = 0, run = 0;
+    __syncthreads();
+    // ---- exclusive scan of the 16 sub-bin counts (one wave)
+    if (wave == 0) {
+      const int v = lane < 16 ? cnt[lane] : 0;
+      int incl = v;
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const uint64_t m = __ballot(valid && sbin == s);
-      cnt_s[s] = (int)__popcll(m);
-      start_s[s] = run;
-      if (valid && sbin == s) pos = run + popc_below(m);
-      run += cnt_s[s];
+      for (int off = 1; off < 16; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+      }
+      if (lane < 16) cnt[lane] = incl - v;  // start of sub-bin lane
+      if (lane == 15) cnt[16] = incl;
     }
-    if (valid) {
-      float* r = stage + pos * REC;
+    __syncthreads();
+    // ---- stage records in sub-bin order
+    if (s >= 0) {
+      float* r = rec + (cnt[s] + slot) * REC;
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
         r[k] = wx[k];
@@ -136,15 +149,16 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
       }
       r[3 * NS] = __int_as_float(offs);
     }
-    wave_lds_fence();
-    // ---- 4 points per MFMA into the sub-bin accumulators
+    __syncthreads();
+    // ---- this wave's 4 sub-bins: 4 points per MFMA into the window accumulators
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      const int n = cnt_s[s];
+    for (int cls = 0; cls < 4; ++cls) {
+      const int sb = owned_subbin(wave, cls);
+      const int st = uniform(cnt[sb]), n = uniform(cnt[sb + 1]) - st;
       for (int g = 0; g < n; g += 4) {
         float are = 0.0f, aim = 0.0f, b = 0.0f;
         if (g + kk < n) {
-          const float* r = stage + (start_s[s] + g + kk) * REC;
+          const float* r = rec + (st + g + kk) * REC;
           const int o = __float_as_int(r[3 * NS]);
           const int iy = rr - (o >> 8), ix = rr - (o & 0xff);
           if ((unsigned)iy < (unsigned)NS) {
@@ -153,42 +167,31 @@ __global__ void __launch_bounds__(64 * kMfmaWaves) k_spread_mfma2d(SIArgs<float>
           }
           if ((unsigned)ix < (unsigned)NS) b = r[ix];
         }
-        acc_re[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(are, b, acc_re[s], 0, 0, 0);
-        acc_im[s] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, b, acc_im[s], 0, 0, 0);
-      }
-    }
-    wave_lds_fence();  // this chunk's record reads before the next chunk's writes
-  }
-  // ---- accumulators -> this wave's tile.  Windows of neighbouring sub-bins overlap,
-  // so lanes of one wave update the same cells; the windows are added in 4 phases
-  // by (sx, sy) parity -- within a phase they are disjoint (2 SB >= 16) -- with a
-  // block barrier between phases (every wave runs the same 4 phases).
-  const int col = lane & 15, row0 = 4 * (lane >> 4);
-#pragma unroll
-  for (int ph = 0; ph < 4; ++ph) {
-#pragma unroll
-    for (int s = 0; s < 16; ++s) {
-      if (((s & 1) | (((s >> 2) & 1) << 1)) != ph) continue;
-      const int x = (s & 3) * SB + col;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int y = (s >> 2) * SB + row0 + r;
-        tre[y * T + x] += acc_re[s][r];
-        tim[y * T + x] += acc_im[s][r];
+        acc_re[cls] = __builtin_amdgcn_mfma_f32_16x16x4f32(are, b, acc_re[cls], 0, 0, 0);
+        acc_im[cls] = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, b, acc_im[cls], 0, 0, 0);
       }
     }
     __syncthreads();
   }
-  // ---- sum the wave tiles, flush to the fine grid
+  // ---- windows -> tile, one parity class at a time (disjoint windows, one per wave)
+  const int col = lane & 15, row0 = 4 * (lane >> 4);
+#pragma unroll
+  for (int cls = 0; cls < 4; ++cls) {
+    const int sb = owned_subbin(wave, cls);
+    const int x = (sb & 3) * SB + col;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int y = (sb >> 2) * SB + row0 + r;
+      tre[y * T + x] += acc_re[cls][r];
+      tim[y * T + x] += acc_im[cls][r];
+    }
+    __syncthreads();
+  }
+  // ---- flush non-zero cells to the fine grid
   cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
   const int tile3[3] = {T, T, 1};
-  for (int cell = threadIdx.x; cell < TC; cell += blockDim.x) {
-    float vr = 0.0f, vi = 0.0f;
-#pragma unroll
-    for (int w = 0; w < kMfmaWaves; ++w) {
-      vr += smem_f[w * G::WAVE_FLOATS + cell];
-      vi += smem_f[w * G::WAVE_FLOATS + TC + cell];
-    }
+  for (int cell = tid; cell < TC; cell += kSpreadThreads) {
+    const float vr = tre[cell], vi = tim[cell];
     if (vr == 0.0f && vi == 0.0f) continue;
     const int64_t gi = tile_cell_to_global<2>(cell, tile3, org, a.nf);
     atomic_add_global(&fT[gi].re, vr);
@@ -243,7 +246,8 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
   const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
   const int tr = blockIdx.y;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = uniform(tid >> 6);
   const int4 sd = a.subprob[sub];
   int org[3];
   tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
@@ -352,15 +356,16 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
     }
     __syncthreads();
     // ---- groups of 16 points: D = T_win Wx on the matrix cores, then the y sum
-    const int ng_total = ngrp;
+    const int ng_total = uniform(ngrp);
     for (int g = wave; g < ng_total; g += kInterpThreads / 64) {
       const int2 gd = grp[g];
-      const int s = gd.y & 255, gn = gd.y >> 8;
+      const int gx = uniform(gd.x), gy = uniform(gd.y);
+      const int s = gy & 255, gn = gy >> 8;
       const bool valid = j < gn;
-      const float* r = rec + (gd.x + min(j, gn - 1)) * REC;
+      const float* r = rec + (gx + min(j, gn - 1)) * REC;
       const int m = __float_as_int(r[2 * NS]);
       const int ox = m & 255, oy = (m >> 8) & 255;
-      const int ox_lo = __shfl(ox, 0, 64), ox_hi = __shfl(ox, gn - 1, 64);
+      const int ox_lo = __builtin_amdgcn_readlane(ox, 0), ox_hi = __builtin_amdgcn_readlane(ox, gn - 1);
       const int xc_lo = ox_lo >> 2, xc_hi = min(3, (ox_hi + NS - 1) >> 2);
       const int rowA = ((s >> 2) * SB + j) * TP + (s & 3) * SB + q;
       mfma_f4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
@@ -389,9 +394,11 @@ __global__ void __launch_bounds__(kInterpThreads) k_interp_mfma2d(SIArgs<float> 
   }
 }
 
-// The matrix-core kernels are opt-in (NUFFTX_MFMA=1) until they beat the LDS-tiled
-// kernels on the 2D benchmark; NUFFTX_NO_MFMA=1 forces them off.
-inline bool use_mfma() {
+// Kernel choice for 2D FP32 ns <= 8: the matrix-core spread is the default (it beats
+// the LDS-tiled fixed-point spread), the matrix-core interpolation is opt-in
+// (NUFFTX_MFMA=1) until it beats the LDS-tiled gather.  NUFFTX_NO_MFMA=1 forces both off.
+inline bool use_mfma_spread() { return std::getenv("NUFFTX_NO_MFMA") == nullptr; }
+inline bool use_mfma_interp() {
   return std::getenv("NUFFTX_MFMA") != nullptr && std::getenv("NUFFTX_NO_MFMA") == nullptr;
 }
 

@@ -2,20 +2,23 @@
 // scatters unsorted, nufftax/core/spread.py:330-334).
 //
 // Contract: perm == argsort(key, stable) and bin_start == [0, cumsum(bincount(key))],
-// where key = sum_d floor(xs_d)/bs_d * prod_{e<d} nb_e and xs is the FP64 fold.
+// where key = sum_d (floor(xs_d) div bs_d) * prod_{e<d} nb_e and xs is the FP64 fold.
 //
 // Counting sort, all integer work:
-//   A  k_hist: one block per chunk of points counts keys in LDS and writes its
-//      row of the histogram H[chunk][bin] (coalesced).
-//   B  segmented column scan of H (k_colsum_seg, k_bin_scan, k_row_offsets):
-//      H[chunk][bin] becomes the first output slot of (bin, chunk), bin_start
-//      the first slot of every bin.
-//   C  k_scatter: the W waves of a chunk block each count their share of the
-//      chunk, turn the counts into per-wave bases, then walk their points in
-//      index order, ranking equal keys inside each 64-point step with one ballot
-//      per key bit, and write perm[slot] = index.
-// A subproblem list (bins split into <= maxsub points) is then built for the
-// tiled spread/interp kernels.
+//   A  k_keys: one block per chunk of points folds each point once, stores its
+//      key and counts the keys in LDS (per-lane integer atomics); the block writes
+//      its row of the histogram H[chunk][bin] (coalesced).
+//   B  segmented column scan of H (k_colsum_seg, k_seg_prefix, k_bin_scan,
+//      k_row_offsets): H[chunk][bin] becomes the first output slot of
+//      (bin, chunk); bin_start the first slot of every bin; sub_start the first
+//      subproblem of every bin.
+//   C  k_scatter: the W waves of a chunk block count their share of the chunk's
+//      keys, turn the counts into per-wave bases, then walk their keys in index
+//      order, ranking equal keys inside each 64-key step (one comparison when the
+//      step's keys agree, else one ballot per key bit), and write perm[slot] = index.
+//   D  k_subprob_fill: descriptors {bin, first sorted slot, count} of the
+//      subproblems (bins split into <= maxsub points) used by the tiled
+//      spread/interp kernels, one thread per descriptor.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -27,8 +30,8 @@ namespace {
 struct KeyArgs {
   const void* coord[3];
   int nf[3];
-  int bs[3];
   int nb[3];
+  double inv_bs[3];  // 1 / bs, rounded
 };
 
 template <typename CT, int D>
@@ -42,17 +45,23 @@ __device__ __forceinline__ int key_of(const KeyArgs& ka, const double (&x)[D]) {
   int key = 0, stride = 1;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    Folded f = fold_coord(x[d], ka.nf[d]);
-    key += (f.cell / ka.bs[d]) * stride;
+    const Folded f = fold_coord(x[d], ka.nf[d]);
+    // cell div bs, exactly: (cell + 1/2) / bs is at least 1/(2 bs) away from an
+    // integer, far more than the rounding error of the product
+    key += (int)(((double)f.cell + 0.5) * ka.inv_bs[d]) * stride;
     stride *= ka.nb[d];
   }
   return key;
 }
 
-// Lanes of the calling wave whose key equals this lane's key (all lanes active),
-// from one ballot per key bit: O(log nbins), no memory traffic.
+// Lanes of the calling wave whose key equals this lane's key (all lanes active).
+// Valid lanes form a prefix of the wave.  When they all hold one key this is a
+// single comparison; otherwise one ballot per key bit, O(log nbins).
 __device__ __forceinline__ uint64_t match_key(int key, bool valid, int key_bits) {
-  uint64_t m = __ballot(valid);
+  const uint64_t vmask = __ballot(valid);
+  const int k0 = __shfl(key, 0, 64);
+  if (__ballot(valid && key != k0) == 0) return vmask;
+  uint64_t m = vmask;
   for (int b = 0; b < key_bits; ++b) {
     const bool bit = (key >> b) & 1;
     const uint64_t mb = __ballot(bit);
@@ -64,10 +73,11 @@ __device__ __forceinline__ uint64_t match_key(int key, bool valid, int key_bits)
 constexpr int kStep = 64;          // points per wave step
 constexpr int kGroup = 8;          // steps whose coordinates are loaded together
 
-// k_hist: block of 64*W threads counts the keys of chunk blockIdx.x ([lo, lo + CH)).
+// k_keys: a block of 64 W threads folds the points of chunk blockIdx.x
+// ([lo, lo + CH)), stores their keys and counts them.
 template <typename CT, int D>
-__global__ void __launch_bounds__(1024) k_hist(KeyArgs ka, int64_t M, int CH, int nbins, int key_bits,
-                                              int32_t* __restrict__ H) {
+__global__ void __launch_bounds__(1024) k_keys(KeyArgs ka, int64_t M, int CH, int nbins, int32_t* __restrict__ keys,
+                                               int32_t* __restrict__ H) {
   extern __shared__ int32_t cnt[];
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
@@ -84,10 +94,11 @@ __global__ void __launch_bounds__(1024) k_hist(KeyArgs ka, int64_t M, int CH, in
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
       const int64_t i = base + (int64_t)u * stride + threadIdx.x;
-      const bool valid = i < hi;
-      const int key = valid ? key_of<D>(ka, x[u]) : 0;
-      const uint64_t m = match_key(key, valid, key_bits);
-      if (valid && popc_below(m) == 0) atomicAdd(&cnt[key], (int)__popcll(m));
+      if (i < hi) {
+        const int key = key_of<D>(ka, x[u]);
+        keys[i] = key;
+        atomicAdd(&cnt[key], 1);
+      }
     }
   }
   __syncthreads();
@@ -95,10 +106,10 @@ __global__ void __launch_bounds__(1024) k_hist(KeyArgs ka, int64_t M, int CH, in
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) row[b] = cnt[b];
 }
 
-// k_scatter: W waves per chunk; wave w owns points [lo + w*SC, lo + (w+1)*SC).
-template <typename CT, int D>
-__global__ void __launch_bounds__(1024) k_scatter(KeyArgs ka, int64_t M, int SC, int nbins, int key_bits,
-                                                 const int32_t* __restrict__ H, int32_t* __restrict__ perm) {
+// k_scatter: W waves per chunk; wave w owns keys [lo + w*SC, lo + (w+1)*SC).
+__global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ keys, int64_t M, int SC, int nbins,
+                                                  int key_bits, const int32_t* __restrict__ H,
+                                                  int32_t* __restrict__ perm) {
   extern __shared__ int32_t wcnt[];  // [W][nbins]
   const int W = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -106,22 +117,17 @@ __global__ void __launch_bounds__(1024) k_scatter(KeyArgs ka, int64_t M, int SC,
   for (int b = lane; b < nbins; b += 64) my[b] = 0;
   const int64_t lo = (int64_t)blockIdx.x * W * SC + (int64_t)wave * SC;
   const int64_t hi = min(M, lo + (int64_t)SC);
-  // count this wave's keys (leaders of equal keys update distinct counters)
+  // count this wave's keys (LDS integer atomics on the wave's own counters)
   for (int64_t base = lo; base < hi; base += kStep * kGroup) {
-    double x[kGroup][D];
+    int k[kGroup];
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
       const int64_t i = base + u * kStep + lane;
-      if (i < hi) load_coords<CT, D>(ka, i, x[u]);
+      k[u] = i < hi ? keys[i] : -1;
     }
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u) {
-      const int64_t i = base + u * kStep + lane;
-      const bool valid = i < hi;
-      const int key = valid ? key_of<D>(ka, x[u]) : 0;
-      const uint64_t m = match_key(key, valid, key_bits);
-      if (valid && popc_below(m) == 0) my[key] += (int)__popcll(m);
-    }
+    for (int u = 0; u < kGroup; ++u)
+      if (k[u] >= 0) atomicAdd(&my[k[u]], 1);
   }
   __syncthreads();
   // per-wave bases: chunk slot of the bin, then earlier waves' counts
@@ -137,17 +143,17 @@ __global__ void __launch_bounds__(1024) k_scatter(KeyArgs ka, int64_t M, int SC,
   __syncthreads();
   // rank in index order and write the permutation
   for (int64_t base = lo; base < hi; base += kStep * kGroup) {
-    double x[kGroup][D];
+    int k[kGroup];
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
       const int64_t i = base + u * kStep + lane;
-      if (i < hi) load_coords<CT, D>(ka, i, x[u]);
+      k[u] = i < hi ? keys[i] : 0;
     }
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
       const int64_t i = base + u * kStep + lane;
       const bool valid = i < hi;
-      const int key = valid ? key_of<D>(ka, x[u]) : 0;
+      const int key = k[u];
       const uint64_t m = match_key(key, valid, key_bits);
       const int rank = popc_below(m);
       const int b0 = my[key];
@@ -200,40 +206,56 @@ __global__ void __launch_bounds__(256) k_colsum_seg(const int32_t* __restrict__ 
   P[(int64_t)seg * nbins + b] = s;
 }
 
-// single block: P[seg][b] -> exclusive prefix over segments, bin totals -> exclusive
-// scan over bins (bin_start); P[seg][b] += bin_start[b].
-__global__ void __launch_bounds__(1024) k_bin_scan(int32_t* __restrict__ P, int nseg, int nbins, int M,
-                                                   int32_t* __restrict__ bin_start) {
-  int carry = 0;
+// P[seg][b] -> exclusive prefix over segments (in place); tot[b] = bin total.
+__global__ void __launch_bounds__(256) k_seg_prefix(int32_t* __restrict__ P, int nseg, int nbins,
+                                                    int32_t* __restrict__ tot) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int run = 0;
+  for (int g = 0; g < nseg; ++g) {
+    const int t = P[(int64_t)g * nbins + b];
+    P[(int64_t)g * nbins + b] = run;
+    run += t;
+  }
+  tot[b] = run;
+}
+
+// single block: bin totals (in bin_start) -> exclusive scan = bin_start; subproblems
+// per bin -> exclusive scan = sub_start; bin_start[nbins] = M; nsub.
+__global__ void __launch_bounds__(1024) k_bin_scan(int32_t* __restrict__ bin_start, int nbins, int M, int maxsub,
+                                                   int max_subprob, int32_t* __restrict__ sub_start,
+                                                   int32_t* __restrict__ nsub) {
+  int carry = 0, carry_sub = 0;
   for (int base = 0; base < nbins; base += 1024) {
     const int b = base + threadIdx.x;
-    int tot = 0;
-    if (b < nbins) {
-      for (int g = 0; g < nseg; ++g) {
-        const int t = P[(int64_t)g * nbins + b];
-        P[(int64_t)g * nbins + b] = tot;
-        tot += t;
-      }
-    }
-    int total;
-    const int ex = block_excl_scan<1024>(tot, total) + carry;
+    const int cnt = b < nbins ? bin_start[b] : 0;
+    const int nsb = (cnt + maxsub - 1) / maxsub;
+    int total, total_sub;
+    const int ex = block_excl_scan<1024>(cnt, total) + carry;
+    const int ex_sub = block_excl_scan<1024>(nsb, total_sub) + carry_sub;
     if (b < nbins) {
       bin_start[b] = ex;
-      for (int g = 0; g < nseg; ++g) P[(int64_t)g * nbins + b] += ex;
+      sub_start[b] = ex_sub;
     }
     carry += total;
+    carry_sub += total_sub;
   }
-  if (threadIdx.x == 0) bin_start[nbins] = M;
+  if (threadIdx.x == 0) {
+    bin_start[nbins] = M;
+    sub_start[nbins] = carry_sub;
+    *nsub = min(carry_sub, max_subprob);
+  }
 }
 
 // H[r][b] -> first output slot of (bin b, chunk r)
 __global__ void __launch_bounds__(256) k_row_offsets(int32_t* __restrict__ H, int NB, int nbins,
-                                                     const int32_t* __restrict__ P) {
+                                                     const int32_t* __restrict__ P,
+                                                     const int32_t* __restrict__ bin_start) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   const int seg = blockIdx.y;
   if (b >= nbins) return;
   const int r0 = seg * kRowSeg, r1 = min(NB, r0 + kRowSeg);
-  int run = P[(int64_t)seg * nbins + b];
+  int run = bin_start[b] + P[(int64_t)seg * nbins + b];
   for (int r = r0; r < r1; ++r) {
     const int t = H[(int64_t)r * nbins + b];
     H[(int64_t)r * nbins + b] = run;
@@ -241,28 +263,24 @@ __global__ void __launch_bounds__(256) k_row_offsets(int32_t* __restrict__ H, in
   }
 }
 
-// single block: subproblem descriptors {bin, first sorted slot, count} from bin_start
-__global__ void __launch_bounds__(1024) k_subprob(const int32_t* __restrict__ bin_start, int nbins, int maxsub,
-                                                  int max_subprob, int4* __restrict__ subprob,
-                                                  int32_t* __restrict__ nsub) {
-  int carry = 0;
-  for (int base = 0; base < nbins; base += 1024) {
-    const int b = base + threadIdx.x;
-    int st = 0, cnt = 0;
-    if (b < nbins) {
-      st = bin_start[b];
-      cnt = bin_start[b + 1] - st;
+// one thread per descriptor slot: its bin is the last b with sub_start[b] <= slot
+__global__ void __launch_bounds__(256) k_subprob_fill(const int32_t* __restrict__ bin_start,
+                                                      const int32_t* __restrict__ sub_start, int nbins, int maxsub,
+                                                      const int32_t* __restrict__ nsub, int4* __restrict__ subprob) {
+  const int n = *nsub;
+  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+    int lo = 0, hi = nbins;  // invariant: sub_start[lo] <= slot < sub_start[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sub_start[mid] <= slot)
+        lo = mid;
+      else
+        hi = mid;
     }
-    const int nsb = (cnt + maxsub - 1) / maxsub;
-    int total;
-    const int ex = block_excl_scan<1024>(nsb, total) + carry;
-    for (int k = 0; k < nsb; ++k) {
-      const int slot = ex + k;
-      if (slot < max_subprob) subprob[slot] = make_int4(b, st + k * maxsub, min(maxsub, cnt - k * maxsub), 0);
-    }
-    carry += total;
+    const int k = slot - sub_start[lo];
+    const int st = bin_start[lo], cnt = bin_start[lo + 1] - st;
+    subprob[slot] = make_int4(lo, st + k * maxsub, min(maxsub, cnt - k * maxsub), 0);
   }
-  if (threadIdx.x == 0) *nsub = min(carry, max_subprob);
 }
 
 __global__ void k_empty_sort(int nbins, int32_t* bin_start, int32_t* nsub) {
@@ -294,15 +312,21 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
   int key_bits = 0;
   while ((1 << key_bits) < nbins) ++key_bits;
   const int CH = W * SC;
-  hipLaunchKernelGGL((k_hist<CT, D>), dim3(ps.NB), dim3(64 * W), (size_t)nbins * sizeof(int32_t), s, ka, ps.M, CH,
-                     nbins, key_bits, ps.hist);
+  hipLaunchKernelGGL((k_keys<CT, D>), dim3(ps.NB), dim3(64 * W), (size_t)nbins * sizeof(int32_t), s, ka, ps.M, CH,
+                     nbins, ps.keys, ps.hist);
   const int nseg = (ps.NB + kRowSeg - 1) / kRowSeg;
   const dim3 g2((nbins + 255) / 256, nseg);
   hipLaunchKernelGGL(k_colsum_seg, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part);
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, ps.scan_part, nseg, nbins, (int)ps.M, ps.bin_start);
-  hipLaunchKernelGGL(k_row_offsets, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part);
-  hipLaunchKernelGGL((k_scatter<CT, D>), dim3(ps.NB), dim3(64 * W), (size_t)W * nbins * sizeof(int32_t), s, ka, ps.M,
+  hipLaunchKernelGGL(k_seg_prefix, dim3((nbins + 255) / 256), dim3(256), 0, s, ps.scan_part, nseg, nbins,
+                     ps.bin_start);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, ps.bin_start, nbins, (int)ps.M, ps.maxsub,
+                     ps.max_subprob, ps.sub_start, ps.nsub);
+  hipLaunchKernelGGL(k_row_offsets, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part, ps.bin_start);
+  hipLaunchKernelGGL(k_scatter, dim3(ps.NB), dim3(64 * W), (size_t)W * nbins * sizeof(int32_t), s, ps.keys, ps.M,
                      SC, nbins, key_bits, ps.hist, ps.perm);
+  const unsigned nfill = (unsigned)std::min<int64_t>((ps.max_subprob + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_subprob_fill, dim3(std::max(1u, nfill)), dim3(256), 0, s, ps.bin_start, ps.sub_start, nbins,
+                     ps.maxsub, ps.nsub, ps.subprob);
 }
 
 }  // namespace
@@ -337,6 +361,8 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   const int64_t nparts = nseg * nbins;
   const int64_t max_sub = nbins + (M + ps.maxsub - 1) / ps.maxsub;
   if ((e = grow(ps.perm, ps.cap_M, M)) != hipSuccess) return e;
+  if ((e = grow(ps.keys, ps.cap_keys, M)) != hipSuccess) return e;
+  if ((e = grow(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1)) != hipSuccess) return e;
   if ((e = grow(ps.hist, ps.cap_hist, nhist)) != hipSuccess) return e;
   if ((e = grow(ps.scan_part, ps.cap_part, nparts)) != hipSuccess) return e;
   if ((e = grow(ps.bin_start, ps.cap_bins, (int64_t)nbins + 1)) != hipSuccess) return e;
@@ -352,8 +378,8 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   for (int d = 0; d < 3; ++d) {
     ka.coord[d] = ps.coord[d];
     ka.nf[d] = g.nf[d];
-    ka.bs[d] = g.bs[d];
     ka.nb[d] = g.nb[d];
+    ka.inv_bs[d] = 1.0 / (double)g.bs[d];
   }
   const int isc = (int)SC;
   if (ps.coord_f64) {
@@ -365,22 +391,22 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
     if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, s);
     if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, s);
   }
-  hipLaunchKernelGGL(k_subprob, dim3(1), dim3(1024), 0, s, ps.bin_start, nbins, ps.maxsub, ps.max_subprob,
-                     ps.subprob, ps.nsub);
   return hipGetLastError();
 }
 
 void pointset_free(PointSet& ps) {
-  void* bufs[] = {ps.perm, ps.hist, ps.scan_part, ps.bin_start, ps.subprob, ps.nsub};
+  void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.bin_start, ps.sub_start, ps.subprob, ps.nsub};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   ps.perm = nullptr;
+  ps.keys = nullptr;
+  ps.sub_start = nullptr;
   ps.hist = nullptr;
   ps.scan_part = nullptr;
   ps.bin_start = nullptr;
   ps.subprob = nullptr;
   ps.nsub = nullptr;
-  ps.cap_M = ps.cap_hist = ps.cap_part = ps.cap_bins = ps.cap_sub = 0;
+  ps.cap_M = ps.cap_keys = ps.cap_hist = ps.cap_part = ps.cap_bins = ps.cap_sub = ps.cap_sub_start = 0;
 }
 
 }  // namespace nufftx

@@ -36,10 +36,8 @@ hipError_t spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_
   auto launch = [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;
     const size_t lds = MfmaGeom<NS>::lds_bytes();
-    auto kern = k_spread_mfma2d<CT, NS>;
-    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(64 * kMfmaWaves), lds, s, a,
-                       static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
+    hipLaunchKernelGGL((k_spread_mfma2d<CT, NS>), dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kSpreadThreads), lds,
+                       s, a, static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
     return hipGetLastError();
   };
   switch (ga.g->ns) {
@@ -54,7 +52,7 @@ hipError_t spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_
 template <typename T, typename CT>
 hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
-    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma())
+    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma_spread())
       return spread_mfma<CT>(ga, c, fw, s);
   }
   switch (ga.g->dim) {
