@@ -192,14 +192,13 @@ def main():
     nfc = nf[0] * nf[1]
     bytes_spread = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c gathered; grid written once
     bytes_interp = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c written; grid read once
-    # 2D FP32 ns <= 8: matrix-core spread by default, matrix-core interp with NUFFTX_MFMA
-    no_mfma = os.environ.get("NUFFTX_NO_MFMA") is not None
-    mfma_interp = os.environ.get("NUFFTX_MFMA") is not None and not no_mfma
+    # 2D FP32 ns <= 8 runs on the matrix cores when NUFFTX_MFMA is set, else on the LDS-tiled kernels
+    mfma = os.environ.get("NUFFTX_MFMA") is not None and os.environ.get("NUFFTX_NO_MFMA") is None
     if t_spread >= t_interp:
-        kname = "k_spread_tiled<float,float,2,8>" if no_mfma else "k_spread_mfma2d<float,8>"
+        kname = "k_spread_mfma2d<8>" if mfma else "k_spread_tiled<float,float,2,8>"
         kt, kb = t_spread, bytes_spread
     else:
-        kname = "k_interp_mfma2d<float,8>" if mfma_interp else "k_gather_tiled<float,float,2,8,false>"
+        kname = "k_interp_mfma2d<8>" if mfma else "k_gather_tiled<float,float,2,8,false>"
         kt, kb = t_interp, bytes_interp
     achieved = kb / kt / 1e9
 

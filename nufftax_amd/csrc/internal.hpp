@@ -45,6 +45,11 @@ struct PointSet {
   int32_t* sub_start = nullptr;  // nbins + 1, first subproblem of every bin
   int4* subprob = nullptr;       // max_subprob
   int32_t* nsub = nullptr;       // 1
+  // 2D FP32 matrix-core geometries: per-subproblem sub-bin order (prep.hip)
+  bool has_recs = false;
+  float4* recs = nullptr;        // M records {t_x, t_y, meta, p}: bin order, then sub-bin order per subproblem
+  int32_t* sub_off = nullptr;    // max_subprob * 17 sub-bin starts per subproblem
+  int64_t cap_recs = 0, cap_sub_off = 0;
   int64_t cap_M = 0, cap_keys = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0, cap_sub_start = 0;
 };
 
@@ -57,9 +62,36 @@ struct SortTuning {
 // Tile size / LDS budget helpers
 void choose_bins(Geometry& g, int prec, int lds_budget);
 int lds_budget_bytes();
+int lds_device_max_bytes();
 
 // ---- sort.hip
-hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err);
+// rec_ns > 0 (2D only): also write the point records of the matrix-core kernels for ns = rec_ns
+hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
+                         int rec_ns = 0);
+
+// device buffer of at least `need` elements (contents not kept when it grows)
+template <typename T>
+hipError_t grow_buffer(T*& p, int64_t& cap, int64_t need) {
+  if (need <= cap && p) return hipSuccess;
+  if (p) {
+    hipError_t e = hipFree(p);
+    if (e != hipSuccess) return e;
+    p = nullptr;
+  }
+  const int64_t n = need > 1 ? need : 1;
+  hipError_t e = hipMalloc((void**)&p, (size_t)n * sizeof(T));
+  if (e != hipSuccess) {
+    cap = 0;
+    p = nullptr;
+    return e;
+  }
+  cap = n;
+  return hipSuccess;
+}
+
+// ---- prep.hip: sub-bin order and point records for the 2D FP32 matrix-core kernels
+bool subsort_applicable(const SortTuning& tune, const Geometry& g, int prec);
+hipError_t pointset_subsort(PointSet& ps, const Geometry& g, hipStream_t s);
 void pointset_free(PointSet& ps);
 
 // ---- spreadinterp.hip

@@ -33,16 +33,21 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
   });
 }
 
-template <typename CT>
-hipError_t interp_mfma(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
-  const SIArgs<float> a = make_si_args<float>(ga);
+// The records-based matrix-core interpolation (needs the setpts point records).
+bool interp_mfma(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s, hipError_t& err) {
   const PointSet& ps = *ga.ps;
-  auto launch = [&](auto nsc) -> hipError_t {
+  if (!ps.has_recs) return false;
+  const SIArgs<float> a = make_si_args<float>(ga);
+  auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;
-    const size_t lds = InterpGeom<NS>::lds_bytes();
-    hipLaunchKernelGGL((k_interp_mfma2d<CT, NS>), dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kInterpThreads), lds,
-                       s, a, static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));
-    return hipGetLastError();
+    const size_t lds = MfmaGeom<NS>::interp_lds();
+    if (lds > (size_t)lds_device_max_bytes()) return false;
+    auto kern = k_interp_mfma2d<NS>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kMfmaThreads), lds, s, a, ps.recs,
+                       ps.sub_off, static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));
+    err = hipGetLastError();
+    return true;
   };
   switch (ga.g->ns) {
     case 2: return launch(std::integral_constant<int, 2>{});
@@ -50,14 +55,14 @@ hipError_t interp_mfma(const GridOpArgs& ga, const void* fw, void* c, hipStream_
     case 6: return launch(std::integral_constant<int, 6>{});
     case 8: return launch(std::integral_constant<int, 8>{});
   }
-  return hipErrorInvalidValue;
+  return false;
 }
 
 template <typename T, typename CT>
 hipError_t interp_dim(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
-    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma_interp())
-      return interp_mfma<CT>(ga, fw, c, s);
+    hipError_t err = hipSuccess;
+    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma() && interp_mfma(ga, fw, c, s, err)) return err;
   }
   switch (ga.g->dim) {
     case 1: return interp_impl<T, CT, 1>(ga, fw, c, s);

@@ -123,6 +123,18 @@ int lds_budget_bytes() {
   return budget;
 }
 
+// Largest dynamic LDS allocation of one block on the current device.
+int lds_device_max_bytes() {
+  static int mx = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      return v;
+    return 65536;
+  }();
+  return mx;
+}
+
 // Bin sizes per dimension: tiles (bin + ns halo) must fit the LDS budget and the
 // bin count the LDS histogram (16384 counters).
 void choose_bins(Geometry& g, int prec, int budget) {
@@ -335,10 +347,16 @@ int check_plan(nufftx_plan p) {
 
 int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g) {
   std::string err;
-  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err);
+  const bool recs = subsort_applicable(p->tune, g, p->prec);
+  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, recs ? g.ns : 0);
   if (e != hipSuccess) {
     if (!err.empty()) return fail(NUFFTX_ERR_INVALID_ARG, err);
     return fail(NUFFTX_ERR_HIP, std::string("bin-sort: ") + hipGetErrorString(e));
+  }
+  ps.has_recs = false;
+  if (recs) {
+    e = pointset_subsort(ps, g, p->stream);
+    if (e != hipSuccess) return fail(NUFFTX_ERR_HIP, std::string("sub-bin sort: ") + hipGetErrorString(e));
   }
   return NUFFTX_OK;
 }

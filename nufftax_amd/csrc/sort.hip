@@ -106,10 +106,36 @@ __global__ void __launch_bounds__(1024) k_keys(KeyArgs ka, int64_t M, int CH, in
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) row[b] = cnt[b];
 }
 
+// Point records of the 2D matrix-core geometries (bins of 4 x 4 sub-bins of
+// SB = 16 - ns cells), written at each point's sorted position:
+//   {t_x, t_y, meta, i}, t = i0 - xs (FP32 from the FP64 fold), meta = ox | oy << 8 |
+//   s << 16 with (ox, oy) the support start inside sub-bin s's 16 x 16 window.
+struct RecArgs {
+  const void* coord[2];
+  int nf[2], bs[2], nb0;
+  float4* recs;
+};
+
+template <typename CT, int NS>
+__device__ __forceinline__ float4 point_record(const RecArgs& ra, int key, double x, double y, int i) {
+  constexpr int SB = 16 - NS;
+  const int bx = key % ra.nb0, by = key / ra.nb0;
+  const int bin_x0 = bx * ra.bs[0], bin_y0 = by * ra.bs[1];
+  const Folded fx = fold_coord(x, ra.nf[0]);
+  const Folded fy = fold_coord(y, ra.nf[1]);
+  const int i0x = support_start(fx, NS / 2), i0y = support_start(fy, NS / 2);
+  const int sx = (fx.cell - bin_x0) / SB, sy = (fy.cell - bin_y0) / SB;
+  const int ox = (i0x - bin_x0 + NS / 2) - sx * SB, oy = (i0y - bin_y0 + NS / 2) - sy * SB;  // in [0, SB]
+  return make_float4((float)((double)i0x - fx.xs), (float)((double)i0y - fy.xs),
+                     __int_as_float(ox | (oy << 8) | ((sy * 4 + sx) << 16)), __int_as_float(i));
+}
+
 // k_scatter: W waves per chunk; wave w owns keys [lo + w*SC, lo + (w+1)*SC).
+// RNS > 0: also write the point records of ns = RNS (coordinates read in index order).
+template <typename CT, int RNS>
 __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ keys, int64_t M, int SC, int nbins,
                                                   int key_bits, const int32_t* __restrict__ H,
-                                                  int32_t* __restrict__ perm) {
+                                                  int32_t* __restrict__ perm, RecArgs ra) {
   extern __shared__ int32_t wcnt[];  // [W][nbins]
   const int W = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -159,7 +185,14 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
       const int b0 = my[key];
       if (valid && rank == 0) my[key] = b0 + (int)__popcll(m);
       const int pos = b0 + rank;
-      if (valid && pos >= 0 && (int64_t)pos < M) perm[pos] = (int32_t)i;
+      if (valid && pos >= 0 && (int64_t)pos < M) {
+        perm[pos] = (int32_t)i;
+        if constexpr (RNS > 0) {
+          const double x = (double)static_cast<const CT*>(ra.coord[0])[i];
+          const double y = (double)static_cast<const CT*>(ra.coord[1])[i];
+          ra.recs[pos] = point_record<CT, RNS>(ra, key, x, y, (int)i);
+        }
+      }
     }
   }
 }
@@ -288,27 +321,9 @@ __global__ void k_empty_sort(int nbins, int32_t* bin_start, int32_t* nsub) {
   if (threadIdx.x == 0) *nsub = 0;
 }
 
-template <typename T>
-hipError_t grow(T*& p, int64_t& cap, int64_t need) {
-  if (need <= cap && p) return hipSuccess;
-  if (p) {
-    hipError_t e = hipFree(p);
-    if (e != hipSuccess) return e;
-    p = nullptr;
-  }
-  int64_t n = std::max<int64_t>(need, 1);
-  hipError_t e = hipMalloc((void**)&p, (size_t)n * sizeof(T));
-  if (e != hipSuccess) {
-    cap = 0;
-    p = nullptr;
-    return e;
-  }
-  cap = n;
-  return hipSuccess;
-}
-
 template <typename CT, int D>
-void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int SC, hipStream_t s) {
+void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int SC, const RecArgs& ra, int rec_ns,
+                        hipStream_t s) {
   int key_bits = 0;
   while ((1 << key_bits) < nbins) ++key_bits;
   const int CH = W * SC;
@@ -322,8 +337,19 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
   hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, ps.bin_start, nbins, (int)ps.M, ps.maxsub,
                      ps.max_subprob, ps.sub_start, ps.nsub);
   hipLaunchKernelGGL(k_row_offsets, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part, ps.bin_start);
-  hipLaunchKernelGGL(k_scatter, dim3(ps.NB), dim3(64 * W), (size_t)W * nbins * sizeof(int32_t), s, ps.keys, ps.M,
-                     SC, nbins, key_bits, ps.hist, ps.perm);
+  const size_t lds = (size_t)W * nbins * sizeof(int32_t);
+  switch (D == 2 ? rec_ns : 0) {
+    case 2: hipLaunchKernelGGL((k_scatter<CT, 2>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
+                               ps.hist, ps.perm, ra); break;
+    case 4: hipLaunchKernelGGL((k_scatter<CT, 4>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
+                               ps.hist, ps.perm, ra); break;
+    case 6: hipLaunchKernelGGL((k_scatter<CT, 6>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
+                               ps.hist, ps.perm, ra); break;
+    case 8: hipLaunchKernelGGL((k_scatter<CT, 8>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
+                               ps.hist, ps.perm, ra); break;
+    default: hipLaunchKernelGGL((k_scatter<CT, 0>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins,
+                                key_bits, ps.hist, ps.perm, ra); break;
+  }
   const unsigned nfill = (unsigned)std::min<int64_t>((ps.max_subprob + 255) / 256, 4096);
   hipLaunchKernelGGL(k_subprob_fill, dim3(std::max(1u, nfill)), dim3(256), 0, s, ps.bin_start, ps.sub_start, nbins,
                      ps.maxsub, ps.nsub, ps.subprob);
@@ -331,7 +357,8 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
 
 }  // namespace
 
-hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err) {
+hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
+                         int rec_ns) {
   hipError_t e;
   const int nbins = g.nbins;
   if ((int64_t)nbins * sizeof(int32_t) > 65536) {
@@ -360,13 +387,14 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   const int64_t nseg = (NB + kRowSeg - 1) / kRowSeg;
   const int64_t nparts = nseg * nbins;
   const int64_t max_sub = nbins + (M + ps.maxsub - 1) / ps.maxsub;
-  if ((e = grow(ps.perm, ps.cap_M, M)) != hipSuccess) return e;
-  if ((e = grow(ps.keys, ps.cap_keys, M)) != hipSuccess) return e;
-  if ((e = grow(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1)) != hipSuccess) return e;
-  if ((e = grow(ps.hist, ps.cap_hist, nhist)) != hipSuccess) return e;
-  if ((e = grow(ps.scan_part, ps.cap_part, nparts)) != hipSuccess) return e;
-  if ((e = grow(ps.bin_start, ps.cap_bins, (int64_t)nbins + 1)) != hipSuccess) return e;
-  if ((e = grow(ps.subprob, ps.cap_sub, max_sub)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.perm, ps.cap_M, M)) != hipSuccess) return e;
+  if (rec_ns > 0 && (e = grow_buffer(ps.recs, ps.cap_recs, M)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.keys, ps.cap_keys, M)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.hist, ps.cap_hist, nhist)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.scan_part, ps.cap_part, nparts)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.bin_start, ps.cap_bins, (int64_t)nbins + 1)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.subprob, ps.cap_sub, max_sub)) != hipSuccess) return e;
   if (!ps.nsub && (e = hipMalloc((void**)&ps.nsub, sizeof(int32_t))) != hipSuccess) return e;
   ps.max_subprob = (int)max_sub;
 
@@ -382,24 +410,38 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
     ka.inv_bs[d] = 1.0 / (double)g.bs[d];
   }
   const int isc = (int)SC;
+  RecArgs ra;
+  ra.coord[0] = ps.coord[0];
+  ra.coord[1] = ps.coord[1];
+  ra.nf[0] = g.nf[0];
+  ra.nf[1] = g.nf[1];
+  ra.bs[0] = g.bs[0];
+  ra.bs[1] = g.bs[1];
+  ra.nb0 = g.nb[0];
+  ra.recs = rec_ns > 0 ? ps.recs : nullptr;
   if (ps.coord_f64) {
-    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, s);
-    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, s);
-    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, W, isc, s);
+    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, ra, 0, s);
+    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, ra, rec_ns, s);
+    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, W, isc, ra, 0, s);
   } else {
-    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, W, isc, s);
-    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, s);
-    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, s);
+    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, W, isc, ra, 0, s);
+    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, ra, rec_ns, s);
+    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, ra, 0, s);
   }
   return hipGetLastError();
 }
 
 void pointset_free(PointSet& ps) {
-  void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.bin_start, ps.sub_start, ps.subprob, ps.nsub};
+  void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.bin_start, ps.sub_start, ps.subprob, ps.nsub,
+                  ps.recs, ps.sub_off};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   ps.perm = nullptr;
   ps.keys = nullptr;
+  ps.recs = nullptr;
+  ps.sub_off = nullptr;
+  ps.has_recs = false;
+  ps.cap_recs = ps.cap_sub_off = 0;
   ps.sub_start = nullptr;
   ps.hist = nullptr;
   ps.scan_part = nullptr;

@@ -29,16 +29,22 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
   });
 }
 
-template <typename CT>
-hipError_t spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
-  const SIArgs<float> a = make_si_args<float>(ga);
+// The records-based matrix-core spread, when the geometry has point records and the
+// kernel's wave tiles fit in LDS; false means "use the LDS-tiled kernel".
+bool spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hipError_t& err) {
   const PointSet& ps = *ga.ps;
-  auto launch = [&](auto nsc) -> hipError_t {
+  if (!ps.has_recs) return false;
+  const SIArgs<float> a = make_si_args<float>(ga);
+  auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;
-    const size_t lds = MfmaGeom<NS>::lds_bytes();
-    hipLaunchKernelGGL((k_spread_mfma2d<CT, NS>), dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kSpreadThreads), lds,
-                       s, a, static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
-    return hipGetLastError();
+    const size_t lds = MfmaGeom<NS>::spread_lds();
+    if (lds > (size_t)lds_device_max_bytes()) return false;
+    auto kern = k_spread_mfma2d<NS>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kMfmaThreads), lds, s, a, ps.recs,
+                       ps.sub_off, static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
+    err = hipGetLastError();
+    return true;
   };
   switch (ga.g->ns) {
     case 2: return launch(std::integral_constant<int, 2>{});
@@ -46,14 +52,14 @@ hipError_t spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_
     case 6: return launch(std::integral_constant<int, 6>{});
     case 8: return launch(std::integral_constant<int, 8>{});
   }
-  return hipErrorInvalidValue;
+  return false;
 }
 
 template <typename T, typename CT>
 hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
-    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma_spread())
-      return spread_mfma<CT>(ga, c, fw, s);
+    hipError_t err = hipSuccess;
+    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma() && spread_mfma(ga, c, fw, s, err)) return err;
   }
   switch (ga.g->dim) {
     case 1: return spread_impl<T, CT, 1>(ga, c, fw, s);

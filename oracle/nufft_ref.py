@@ -604,3 +604,4 @@ def bin_sort(keys: np.ndarray, n_bins: int):
     perm = np.argsort(keys, kind="stable").astype(np.int64)
     offsets = np.concatenate([[0], np.cumsum(np.bincount(keys, minlength=n_bins))]).astype(np.int64)
     return perm, offsets
+

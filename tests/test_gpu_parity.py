@@ -309,14 +309,13 @@ def test_type3_vs_direct(rng, dim, prec):
 
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
-@pytest.mark.parametrize("path", ["default", "no-mfma"])
-def test_spread2d_f32_every_nspread(rng, eps, dist, path, monkeypatch):
-    """2D FP32 spreading (matrix-core kernel for ns <= 8 by default, and the LDS-tiled kernel) vs the FP64
-    oracle, every even ns."""
+@pytest.mark.parametrize("mfma", ["1", None])
+def test_spread2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
+    """2D FP32 spreading (LDS-tiled, and the matrix-core path for ns <= 8) vs the FP64 oracle, every even ns."""
     from nufftax_amd._plan import F32, grid_plan
 
-    if path == "no-mfma":
-        monkeypatch.setenv("NUFFTX_NO_MFMA", "1")
+    if mfma:
+        monkeypatch.setenv("NUFFTX_MFMA", mfma)
     kp = R.compute_kernel_params(eps)
     nfs = (4 * (16 - kp.nspread) * 3 + 5, 4 * (16 - kp.nspread) * 2 + 3)  # several bins, ragged edge
     M = 20000

@@ -335,3 +335,4 @@ def test_bin_keys_and_sort(rng):
     for b in range(12):
         seg = perm[offs[b]:offs[b + 1]]
         assert np.all(keys[seg] == b) and np.all(np.diff(seg) > 0)  # stable
+
