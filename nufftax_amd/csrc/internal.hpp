@@ -49,7 +49,8 @@ struct PointSet {
   bool has_recs = false;
   float4* recs = nullptr;        // M records {t_x, t_y, meta, p}: bin order, then sub-bin order per subproblem
   int32_t* sub_off = nullptr;    // max_subprob * 17 sub-bin starts per subproblem
-  int64_t cap_recs = 0, cap_sub_off = 0;
+  float* cmax = nullptr;          // n_trans: max |re|, |im| of the strengths (matrix-core spread)
+  int64_t cap_recs = 0, cap_sub_off = 0, cap_cmax = 0;
   int64_t cap_M = 0, cap_keys = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0, cap_sub_start = 0;
 };
 

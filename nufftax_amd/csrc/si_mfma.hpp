@@ -48,9 +48,9 @@ struct MfmaGeom {
   static constexpr int SREC = 3 * NS + 1;  // spread stage per point: wx[NS], (c wy)[NS] complex, meta
   static constexpr int IREC = 18;          // interp stage per point: wx[8], wy[8], meta, point index
   static constexpr int WAVES = kMfmaThreads / 64;
-  // spreading: the block's 64-bit fixed-point tile (re, im) and a 64-point stage per wave
+  // spreading: the block's 64-bit fixed-point tile (re, im; row stride TP) and a 64-point stage per wave
   static constexpr size_t spread_lds() {
-    return 2 * (size_t)TC * sizeof(unsigned long long) + (size_t)WAVES * 64 * SREC * sizeof(float);
+    return 2 * (size_t)T * TP * sizeof(unsigned long long) + (size_t)WAVES * 64 * SREC * sizeof(float);
   }
   // interpolation: the tile plus a 64-point stage per wave
   static constexpr size_t interp_lds() { return (2 * (size_t)T * TP + (size_t)WAVES * 64 * IREC) * sizeof(float); }
@@ -117,29 +117,30 @@ struct GroupWalk {
 // one pair of MFMAs (re, im) into the window accumulator, lane (i, k) feeding
 // A[i][k] = c_k wy_k[i - oy_k] and B[k][i] = wx_k[i - ox_k].  Whenever the sub-bin
 // changes the window is added into the block's tile in exact 64-bit fixed point
-// (integer LDS atomics, fast on gfx950; scale 2^(50 - e) with sum |c| < 2^e over the
-// subproblem, as in the LDS-tiled kernel), so the tile costs 16 B per cell and no
+// (integer LDS atomics; scale 2^(50 - e) with 2 n max|c| < 2^e, max|c| over the
+// transform from one coalesced reduction), so the tile costs 16 B per cell and no
 // wave-private copies: occupancy is what hides this kernel's latencies.
 template <int NS>
 __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a, const float4* __restrict__ recs,
                                                                  const int32_t* __restrict__ sub_off,
+                                                                 const float* __restrict__ cmax,
                                                                  const cplx<float>* __restrict__ c,
                                                                  cplx<float>* __restrict__ fw) {
   using G = MfmaGeom<NS>;
   using Word = unsigned long long;
-  constexpr int SB = G::SB, T = G::T, TC = G::TC, SREC = G::SREC;
+  constexpr int SB = G::SB, T = G::T, TP = G::TP, SREC = G::SREC;
+  constexpr int TCP = T * TP;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Word* sre = reinterpret_cast<Word*>(smem_raw);
-  Word* sim = sre + TC;
+  Word* sim = sre + TCP;
   __shared__ int32_t soff[17];
-  __shared__ double red[G::WAVES];
   const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
   const int tr = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = uniform(tid >> 6);
-  float* stage = reinterpret_cast<float*>(sim + TC) + wave * 64 * SREC;
-  for (int e = tid; e < TC; e += kMfmaThreads) {
+  float* stage = reinterpret_cast<float*>(sim + TCP) + wave * 64 * SREC;
+  for (int e = tid; e < TCP; e += kMfmaThreads) {
     sre[e] = 0;
     sim[e] = 0;
   }
@@ -149,19 +150,9 @@ __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a,
   tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
   const cplx<float>* cT = c + (int64_t)tr * a.M;
   const float4* rT = recs + sd.y;
-  // ---- fixed-point scale of this subproblem
-  double part = 0.0;
-  for (int jj = tid; jj < sd.z; jj += kMfmaThreads) {
-    const cplx<float> cv = cT[__float_as_int(rT[jj].w)];
-    part += fabs((double)cv.re) + fabs((double)cv.im);
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-  if (lane == 0) red[wave] = part;
   __syncthreads();
-  double B = 0.0;
-#pragma unroll
-  for (int w = 0; w < G::WAVES; ++w) B += red[w];
+  // ---- fixed-point scale: every partial sum is below sum_j (|re| + |im|) <= 2 n max|c|
+  const double B = 2.0 * (double)sd.z * (double)cmax[tr];
   const bool finite = isfinite(B);
   double scale = 1.0, inv_scale = 1.0;
   if (finite && B > 0.0) {
@@ -181,8 +172,8 @@ __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a,
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
       const int y = (sbin >> 2) * SB + 4 * k + rr;
-      if (acc_re[rr] != 0.0f) atomicAdd(&sre[y * T + x], TileAcc<float>::quant((double)acc_re[rr], scale));
-      if (acc_im[rr] != 0.0f) atomicAdd(&sim[y * T + x], TileAcc<float>::quant((double)acc_im[rr], scale));
+      if (acc_re[rr] != 0.0f) atomicAdd(&sre[y * TP + x], TileAcc<float>::quant((double)acc_re[rr], scale));
+      if (acc_im[rr] != 0.0f) atomicAdd(&sim[y * TP + x], TileAcc<float>::quant((double)acc_im[rr], scale));
     }
   };
   for (int gb = gw.g0; gb < gw.g1; gb += 16) {
@@ -239,8 +230,9 @@ __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a,
   // ---- flush non-zero cells to the fine grid
   cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
   const int tile3[3] = {T, T, 1};
-  for (int cell = tid; cell < TC; cell += kMfmaThreads) {
-    const long long ir = (long long)sre[cell], ii = (long long)sim[cell];
+  for (int cell = tid; cell < T * T; cell += kMfmaThreads) {
+    const int l = (cell / T) * TP + cell % T;
+    const long long ir = (long long)sre[l], ii = (long long)sim[l];
     if (finite && ir == 0 && ii == 0) continue;
     const float vr = finite ? (float)((double)ir * inv_scale) : NAN;
     const float vi = finite ? (float)((double)ii * inv_scale) : NAN;
@@ -248,6 +240,25 @@ __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a,
     atomic_add_global(&fT[gi].re, vr);
     atomic_add_global(&fT[gi].im, vi);
   }
+}
+
+// cmax[t] = max over j of max(|re c[t][j]|, |im c[t][j]|) (float bits; cmax zeroed by the caller;
+// a NaN or infinity propagates as the largest bit pattern).  One global atomic per block.
+static __global__ void __attribute__((unused)) __launch_bounds__(256) k_absmax(const cplx<float>* __restrict__ c,
+                                                                              int64_t M, float* __restrict__ cmax) {
+  __shared__ unsigned wmax[4];
+  const cplx<float>* cT = c + (int64_t)blockIdx.y * M;
+  unsigned m = 0;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (int64_t)gridDim.x * blockDim.x) {
+    const cplx<float> v = cT[j];
+    m = max(m, max(__float_as_uint(fabsf(v.re)), __float_as_uint(fabsf(v.im))));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(reinterpret_cast<unsigned*>(cmax) + blockIdx.y, max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])));
 }
 
 // ---------------------------------------------------------------- interpolation

@@ -433,15 +433,16 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
 
 void pointset_free(PointSet& ps) {
   void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.bin_start, ps.sub_start, ps.subprob, ps.nsub,
-                  ps.recs, ps.sub_off};
+                  ps.recs, ps.sub_off, ps.cmax};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   ps.perm = nullptr;
   ps.keys = nullptr;
   ps.recs = nullptr;
   ps.sub_off = nullptr;
+  ps.cmax = nullptr;
   ps.has_recs = false;
-  ps.cap_recs = ps.cap_sub_off = 0;
+  ps.cap_recs = ps.cap_sub_off = ps.cap_cmax = 0;
   ps.sub_start = nullptr;
   ps.hist = nullptr;
   ps.scan_part = nullptr;

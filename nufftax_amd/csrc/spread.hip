@@ -1,4 +1,5 @@
 // Type-1 spreading launchers (kernels in si_kernels.hpp).
+#include <algorithm>
 #include <cstdlib>
 
 #include "si_mfma.hpp"
@@ -32,17 +33,22 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
 // The records-based matrix-core spread, when the geometry has point records and the
 // kernel's wave tiles fit in LDS; false means "use the LDS-tiled kernel".
 bool spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hipError_t& err) {
-  const PointSet& ps = *ga.ps;
+  PointSet& ps = const_cast<PointSet&>(*ga.ps);
   if (!ps.has_recs) return false;
   const SIArgs<float> a = make_si_args<float>(ga);
   auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;
     const size_t lds = MfmaGeom<NS>::spread_lds();
     if (lds > (size_t)lds_device_max_bytes()) return false;
+    if ((err = grow_buffer(ps.cmax, ps.cap_cmax, ga.n_trans)) != hipSuccess) return true;
+    if ((err = launch_zero(ps.cmax, ga.n_trans * (int64_t)sizeof(float), s)) != hipSuccess) return true;
+    const unsigned nred = (unsigned)std::min<int64_t>((ps.M + 255) / 256, 512);
+    hipLaunchKernelGGL(k_absmax, dim3(nred, (unsigned)ga.n_trans), dim3(256), 0, s,
+                       static_cast<const cplx<float>*>(c), ps.M, ps.cmax);
     auto kern = k_spread_mfma2d<NS>;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kMfmaThreads), lds, s, a, ps.recs,
-                       ps.sub_off, static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
+                       ps.sub_off, ps.cmax, static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
     err = hipGetLastError();
     return true;
   };
