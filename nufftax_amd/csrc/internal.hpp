@@ -45,9 +45,11 @@ struct PointSet {
   int32_t* sub_start = nullptr;  // nbins + 1, first subproblem of every bin
   int4* subprob = nullptr;       // max_subprob
   int32_t* nsub = nullptr;       // 1
-  // 2D FP32 matrix-core geometries: per-subproblem sub-bin order (prep.hip)
+  // 2D FP32 point sets: point records in sorted order (sort.hip), optionally re-ordered
+  // by sub-bin inside every subproblem for the matrix-core kernels (prep.hip)
   bool has_recs = false;
-  float4* recs = nullptr;        // M records {t_x, t_y, meta, p}: bin order, then sub-bin order per subproblem
+  bool recs_subbin = false;
+  float4* recs = nullptr;        // M records {t_x, t_y, meta, p}
   int32_t* sub_off = nullptr;    // max_subprob * 17 sub-bin starts per subproblem
   float* cmax = nullptr;          // n_trans: max |re|, |im| of the strengths (matrix-core spread)
   int64_t cap_recs = 0, cap_sub_off = 0, cap_cmax = 0;
@@ -66,9 +68,10 @@ int lds_budget_bytes();
 int lds_device_max_bytes();
 
 // ---- sort.hip
-// rec_ns > 0 (2D only): also write the point records of the matrix-core kernels for ns = rec_ns
+// want_recs (2D only): also write the point records; sub_sb: sub-bin side of the
+// matrix-core geometry (16 - ns) to record sub-bins in them, 0 otherwise
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
-                         int rec_ns = 0);
+                         bool want_recs = false, int sub_sb = 0);
 
 // device buffer of at least `need` elements (contents not kept when it grows)
 template <typename T>

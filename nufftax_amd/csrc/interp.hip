@@ -21,6 +21,8 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
     constexpr int NS = decltype(nsc)::value;
     if (tiled) {
       auto kern = k_gather_tiled<T, CT, D, NS, false>;
+      if constexpr (D == 2 && std::is_same<T, float>::value)
+        if (a.recs) kern = k_gather_tiled<T, CT, D, NS, false, true>;
       if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
                          static_cast<const cplx<T>*>(fw), o);
@@ -36,7 +38,7 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
 // The records-based matrix-core interpolation (needs the setpts point records).
 bool interp_mfma(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s, hipError_t& err) {
   const PointSet& ps = *ga.ps;
-  if (!ps.has_recs) return false;
+  if (!ps.recs_subbin) return false;
   const SIArgs<float> a = make_si_args<float>(ga);
   auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;

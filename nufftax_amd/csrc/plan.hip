@@ -345,16 +345,18 @@ int check_plan(nufftx_plan p) {
   return NUFFTX_OK;
 }
 
-int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g) {
+// interp: the point set is interpolated to (type 2, grid operators, type-3 targets);
+// only then do the point records pay for their cost in the bin-sort.
+int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g, bool interp) {
   std::string err;
-  const bool recs = subsort_applicable(p->tune, g, p->prec);
-  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, recs ? g.ns : 0);
+  const bool subsort = subsort_applicable(p->tune, g, p->prec);
+  const bool recs = g.dim == 2 && p->prec == kF32 && (interp || subsort);
+  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, recs, subsort ? 16 - g.ns : 0);
   if (e != hipSuccess) {
     if (!err.empty()) return fail(NUFFTX_ERR_INVALID_ARG, err);
     return fail(NUFFTX_ERR_HIP, std::string("bin-sort: ") + hipGetErrorString(e));
   }
-  ps.has_recs = false;
-  if (recs) {
+  if (subsort && ps.has_recs) {
     e = pointset_subsort(ps, g, p->stream);
     if (e != hipSuccess) return fail(NUFFTX_ERR_HIP, std::string("sub-bin sort: ") + hipGetErrorString(e));
   }
@@ -581,7 +583,7 @@ int nufftx_setpts(nufftx_plan p, int64_t M, const void* x, const void* y, const 
     p->ps.M = M;
     for (int d = 0; d < 3; ++d) p->ps.coord[d] = d < p->dim ? xs[d] : nullptr;
     p->ps.coord_f64 = p->prec == kF64;
-    if (int r = sort_points(p, p->ps, p->g)) return r;
+    if (int r = sort_points(p, p->ps, p->g, p->type != 1)) return r;
     p->pts_set = true;
     return NUFFTX_OK;
   }
@@ -616,8 +618,8 @@ int nufftx_setpts(nufftx_plan p, int64_t M, const void* x, const void* y, const 
     p->pt.coord[d] = d < p->dim ? p->sr[d] : nullptr;
   }
   p->ps.coord_f64 = p->pt.coord_f64 = true;
-  if ((r = sort_points(p, p->ps, p->g))) return r;
-  if ((r = sort_points(p, p->pt, p->gi))) return r;
+  if ((r = sort_points(p, p->ps, p->g, false))) return r;
+  if ((r = sort_points(p, p->pt, p->gi, true))) return r;
   p->pts_set = true;
   return NUFFTX_OK;
 }

@@ -2,8 +2,8 @@
 //
 // The bin-sort's scatter writes, at every sorted position, the record
 //   {t_x, t_y, meta, p}:  t = i0 - xs (FP32, from the FP64 fold; i0 = ceil(xs - ns/2),
-//   nufftax/core/spread.py:216-230), meta = ox | oy << 8 | s << 16 with (ox, oy) the
-//   support start inside the sub-bin's 16 x 16 window and s the sub-bin, p the point.
+//   nufftax/core/spread.py:216-230), meta = ox | oy << 12 | s << 24 with (ox, oy) the
+//   support start in the bin's tile and s the sub-bin, p the point.
 // k_subsort then re-orders every subproblem's slice of records by sub-bin (stable:
 // bin-sort order within a sub-bin), reading and writing only that slice, and
 // records sub_off[sub][s], the first record of sub-bin s (sub_off[sub][16] = count).
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_subsort(PrepArgs a) {
     const int j = u * kPrepThreads + tid;
     if (j < sd.z) {
       r[u] = a.recs[sd.y + j];
-      skey[j] = (unsigned char)((__float_as_int(r[u].z) >> 16) & 15);
+      skey[j] = (unsigned char)((__float_as_int(r[u].z) >> 24) & 15);
     }
   }
   __syncthreads();
@@ -118,8 +118,8 @@ bool subsort_applicable(const SortTuning& tune, const Geometry& g, int prec) {
 
 hipError_t pointset_subsort(PointSet& ps, const Geometry& g, hipStream_t s) {
   (void)g;
-  ps.has_recs = false;
-  if (ps.M == 0) return hipSuccess;
+  ps.recs_subbin = false;
+  if (ps.M == 0 || !ps.has_recs) return hipSuccess;
   hipError_t e;
   if ((e = grow_buffer(ps.sub_off, ps.cap_sub_off, (int64_t)ps.max_subprob * 17)) != hipSuccess) return e;
   PrepArgs a;
@@ -129,7 +129,7 @@ hipError_t pointset_subsort(PointSet& ps, const Geometry& g, hipStream_t s) {
   a.sub_off = ps.sub_off;
   hipLaunchKernelGGL(k_subsort, dim3((unsigned)std::max(1, ps.max_subprob)), dim3(kPrepThreads), 0, s, a);
   e = hipGetLastError();
-  if (e == hipSuccess) ps.has_recs = true;
+  if (e == hipSuccess) ps.recs_subbin = true;
   return e;
 }
 

@@ -22,6 +22,8 @@ namespace nufftx {
 template <typename T>
 struct SIArgs {
   const int32_t* perm;
+  const float4* recs;  // 2D FP32 point records in sorted order, or null (sort.hip)
+  const float* cmax;   // FP32 spreading: max |re|, |im| of the strengths per transform
   const int4* subprob;
   const int32_t* nsub;
   const void* coord[3];
@@ -135,9 +137,9 @@ __device__ __forceinline__ void rot_weights(const SIArgs<T>& a, int d, int p, in
 // its integer ones, so FP32 plans accumulate each tile in exact 64-bit fixed
 // point: a contribution v becomes round(v * 2^s) through one FP64 FMA with the
 // constant 1.5 * 2^52 (the rounded integer lands in the low mantissa bits), and
-// is added with ds_add_u64.  s is chosen per subproblem from B = sum |Re c| + |Im c|
-// of its points so that every partial sum stays below 2^50; the tile sum is then
-// exact and independent of the order of the atomics.  FP64 plans use ds_add_f64.
+// is added with ds_add_u64.  s is chosen per subproblem from a bound B >= sum |Re c| +
+// |Im c| of its points so that every partial sum stays below 2^50; the tile sum is
+// then exact and independent of the order of the atomics.  FP64 plans use ds_add_f64.
 template <typename T>
 struct TileAcc;
 
@@ -151,7 +153,9 @@ struct TileAcc<float> {
   }
 };
 
-template <typename T, typename CT, int D, int NS>
+// REC (2D FP32): points come from the sorted point records (coalesced) instead of
+// perm + a gather and refold of the coordinates.
+template <typename T, typename CT, int D, int NS, bool REC = false>
 __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>* __restrict__ c,
                                                       cplx<T>* __restrict__ fw) {
   constexpr bool kFixed = std::is_same<T, float>::value;
@@ -159,7 +163,6 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Acc* sre = reinterpret_cast<Acc*>(smem_raw);
   Acc* sim = sre + a.tile_cells;
-  __shared__ double red[4];
   const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
   const int tr = blockIdx.y;
@@ -171,22 +174,13 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
     sim[i] = (Acc)0;
   }
   const cplx<T>* cT = c + (int64_t)tr * a.M;
-  // fixed-point scale of this subproblem
+  // fixed-point scale of this subproblem: every partial sum of a tile cell is below
+  // B = sum_j (|re c_j| + |im c_j|) <= 2 n max|c| (max over the transform, one
+  // coalesced reduction before the launch; weights are at most 1)
   double scale = 1.0, inv_scale = 1.0;
   bool finite = true;
   if constexpr (kFixed) {
-    double part = 0.0;
-    for (int j = threadIdx.x; j < sd.z; j += blockDim.x) {
-      const int p = a.perm[sd.y + j];
-      if ((unsigned)p >= (unsigned)a.M) continue;
-      const cplx<T> cv = cT[p];
-      part += fabs((double)cv.re) + fabs((double)cv.im);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
-    __syncthreads();
-    const double B = red[0] + red[1] + red[2] + red[3];
+    const double B = 2.0 * (double)sd.z * (double)a.cmax[tr];
     finite = isfinite(B);
     if (finite && B > 0.0) {
       int e;
@@ -200,14 +194,29 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
   int rot[3];
   lane_rotation<NS>(rot);
   for (int j = threadIdx.x; j < sd.z && finite; j += blockDim.x) {
-    const int p = a.perm[sd.y + j];
+    float4 rc;
+    int p;
+    if constexpr (REC) {
+      rc = a.recs[sd.y + j];
+      p = __float_as_int(rc.w);
+    } else {
+      p = a.perm[sd.y + j];
+    }
     if ((unsigned)p >= (unsigned)a.M) continue;
     const cplx<T> cv = cT[p];
     T w0[NS], w1[NS];
     int i00, i01 = 0;
     T tx, ty;
-    rot_weights<T, CT, NS>(a, 0, p, rot[0], w0, i00, tx);
-    const int l0 = i00 - org[0];
+    int l0;
+    if constexpr (REC) {
+      l0 = __float_as_int(rc.z) & 4095;
+      tx = (T)rc.x;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) w0[k] = es_phi(tx + (T)rot_off<NS>(k, rot[0]), a.es.beta_log2e, a.es.c);
+    } else {
+      rot_weights<T, CT, NS>(a, 0, p, rot[0], w0, i00, tx);
+      l0 = i00 - org[0];
+    }
     int o0[NS];
     double wd[NS];
 #pragma unroll
@@ -230,8 +239,16 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
 #pragma unroll
       for (int k = 0; k < NS; ++k) put(o0[k], cre, cim, k);
     } else {
-      rot_weights<T, CT, NS>(a, 1, p, rot[1], w1, i01, ty);
-      const int l1 = i01 - org[1];
+      int l1;
+      if constexpr (REC) {
+        l1 = (__float_as_int(rc.z) >> 12) & 4095;
+        ty = (T)rc.y;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) w1[k] = es_phi(ty + (T)rot_off<NS>(k, rot[1]), a.es.beta_log2e, a.es.c);
+      } else {
+        rot_weights<T, CT, NS>(a, 1, p, rot[1], w1, i01, ty);
+        l1 = i01 - org[1];
+      }
       if constexpr (D == 2) {
 #pragma unroll
         for (int ky = 0; ky < NS; ++ky) {
@@ -443,7 +460,8 @@ __device__ __forceinline__ void emit_point(const SIArgs<T>& a, const GatherOut<T
 }
 
 // ------------------------------------------------------------------ interp / grad (tiled)
-template <typename T, typename CT, int D, int NS, bool DERIV>
+// REC (2D FP32, values only): points come from the sorted point records.
+template <typename T, typename CT, int D, int NS, bool DERIV, bool REC = false>
 __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>* __restrict__ fw,
                                                       GatherOut<T, D, DERIV> o) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -460,17 +478,31 @@ __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>
   __syncthreads();
   const int t0 = a.tile[0], t1 = a.tile[1];
   for (int j = threadIdx.x; j < sd4.z; j += blockDim.x) {
-    const int p = a.perm[sd4.y + j];
-    if ((unsigned)p >= (unsigned)a.M) continue;
     T w[D][NS], dw[D][NS];
-    int i0[D];
-    if constexpr (DERIV)
-      point_weights_deriv<T, CT, D, NS>(a, p, w, dw, i0);
-    else
-      point_weights<T, CT, D, NS>(a, p, w, i0);
     int l[3] = {0, 0, 0};
+    int p;
+    if constexpr (REC) {
+      const float4 rc = a.recs[sd4.y + j];
+      p = __float_as_int(rc.w);
+      const int meta = __float_as_int(rc.z);
+      l[0] = meta & 4095;
+      l[1] = (meta >> 12) & 4095;
 #pragma unroll
-    for (int d = 0; d < D; ++d) l[d] = i0[d] - org[d];
+      for (int k = 0; k < NS; ++k) {
+        w[0][k] = es_phi((T)rc.x + (T)k, a.es.beta_log2e, a.es.c);
+        w[D - 1][k] = es_phi((T)rc.y + (T)k, a.es.beta_log2e, a.es.c);
+      }
+    } else {
+      p = a.perm[sd4.y + j];
+      if ((unsigned)p >= (unsigned)a.M) continue;
+      int i0[D];
+      if constexpr (DERIV)
+        point_weights_deriv<T, CT, D, NS>(a, p, w, dw, i0);
+      else
+        point_weights<T, CT, D, NS>(a, p, w, i0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) l[d] = i0[d] - org[d];
+    }
     auto fetch = [&](int kz, int ky, int kx) -> cplx<T> {
       return tile[((int64_t)(l[2] + kz) * t1 + (l[1] + ky)) * t0 + l[0] + kx];
     };
@@ -528,6 +560,8 @@ SIArgs<T> make_si_args(const GridOpArgs& ga) {
   const Geometry& g = *ga.g;
   SIArgs<T> a;
   a.perm = ps.perm;
+  a.recs = (std::is_same<T, float>::value && g.dim == 2 && ps.has_recs) ? ps.recs : nullptr;
+  a.cmax = nullptr;
   a.subprob = ps.subprob;
   a.nsub = ps.nsub;
   for (int d = 0; d < 3; ++d) {

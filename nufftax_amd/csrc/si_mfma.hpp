@@ -9,9 +9,9 @@
 // v_mfma_f32_16x16x4f32 accumulates 4 points into a window (spreading) or
 // contracts a window with the x-weights of 16 points (interpolation).
 //
-// Both kernels read the setpts-time point records (prep.hip): each subproblem's
-// points in sub-bin order with their window offsets and t = i0 - xs, so no point
-// is refolded here.  A block of 4 waves takes one subproblem; its groups (4 or 16
+// Both kernels read the setpts-time point records (sort.hip, prep.hip): each
+// subproblem's points in sub-bin order with their tile offsets and t = i0 - xs, so
+// no point is refolded here.  A block of 4 waves takes one subproblem; its groups (4 or 16
 // points of one sub-bin) are split into 4 contiguous ranges, one per wave, so the
 // waves stay balanced whatever the distribution of points over sub-bins.
 // Spreading keeps one window accumulator in registers and adds it into the block's
@@ -213,7 +213,8 @@ __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a,
       const int meta = __float_as_int(r[3 * NS]);
       float are = 0.0f, aim = 0.0f, b = 0.0f;
       if (meta >= 0) {
-        const int iy = i - ((meta >> 8) & 255), ix = i - (meta & 255);
+        const int sbm = (meta >> 24) & 15;
+        const int iy = i - (((meta >> 12) & 4095) - (sbm >> 2) * SB), ix = i - ((meta & 4095) - (sbm & 3) * SB);
         if ((unsigned)iy < (unsigned)NS) {
           are = r[NS + 2 * iy];
           aim = r[NS + 2 * iy + 1];
@@ -242,14 +243,26 @@ __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a,
   }
 }
 
-// cmax[t] = max over j of max(|re c[t][j]|, |im c[t][j]|) (float bits; cmax zeroed by the caller;
-// a NaN or infinity propagates as the largest bit pattern).  One global atomic per block.
-static __global__ void __attribute__((unused)) __launch_bounds__(256) k_absmax(const cplx<float>* __restrict__ c,
-                                                                              int64_t M, float* __restrict__ cmax) {
+// cmax[t] = max over j of max(|re c[t][j]|, |im c[t][j]|), as float bits (a NaN or an
+// infinity propagates as the largest pattern): per-block maxima into part[t][block],
+// then one block per transform reduces them.  No atomics.
+constexpr int kAbsmaxBlocks = 1024;
+
+static __global__ void __attribute__((unused)) __launch_bounds__(256) k_absmax_part(const cplx<float>* __restrict__ c,
+                                                                                   int64_t M, unsigned* __restrict__ part) {
   __shared__ unsigned wmax[4];
   const cplx<float>* cT = c + (int64_t)blockIdx.y * M;
   unsigned m = 0;
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; j + 3 * stride < M; j += 4 * stride) {
+    cplx<float> v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = cT[j + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) m = max(m, max(__float_as_uint(fabsf(v[u].re)), __float_as_uint(fabsf(v[u].im))));
+  }
+  for (; j < M; j += stride) {
     const cplx<float> v = cT[j];
     m = max(m, max(__float_as_uint(fabsf(v.re)), __float_as_uint(fabsf(v.im))));
   }
@@ -257,8 +270,19 @@ static __global__ void __attribute__((unused)) __launch_bounds__(256) k_absmax(c
   for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
   if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0)
-    atomicMax(reinterpret_cast<unsigned*>(cmax) + blockIdx.y, max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])));
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+}
+
+static __global__ void __attribute__((unused)) __launch_bounds__(256) k_absmax_final(const unsigned* __restrict__ part,
+                                                                                    int nparts, float* __restrict__ cmax) {
+  __shared__ unsigned wmax[4];
+  unsigned m = 0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = max(m, part[(int64_t)blockIdx.x * nparts + b]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) cmax[blockIdx.x] = __uint_as_float(max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3])));
 }
 
 // ---------------------------------------------------------------- interpolation
@@ -335,7 +359,7 @@ __global__ void __launch_bounds__(kMfmaThreads) k_interp_mfma2d(SIArgs<float> a,
       const int s = gw.s;
       const float* r = stage + (16 * gl + j) * IREC;
       const int meta = __float_as_int(r[16]);
-      const int ox = meta & 255, oy = (meta >> 8) & 255;
+      const int ox = (meta & 4095) - (s & 3) * SB, oy = ((meta >> 12) & 4095) - (s >> 2) * SB;
       const int rowA = ((s >> 2) * SB + j) * TP + (s & 3) * SB + q;
       mfma_f4 dre = {0.f, 0.f, 0.f, 0.f}, dim = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

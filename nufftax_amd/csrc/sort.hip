@@ -106,33 +106,35 @@ __global__ void __launch_bounds__(1024) k_keys(KeyArgs ka, int64_t M, int CH, in
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) row[b] = cnt[b];
 }
 
-// Point records of the 2D matrix-core geometries (bins of 4 x 4 sub-bins of
-// SB = 16 - ns cells), written at each point's sorted position:
-//   {t_x, t_y, meta, i}, t = i0 - xs (FP32 from the FP64 fold), meta = ox | oy << 8 |
-//   s << 16 with (ox, oy) the support start inside sub-bin s's 16 x 16 window.
+// Point records of 2D FP32 point sets, written at each point's sorted position so
+// that the spread/interp kernels read them contiguously instead of gathering the
+// coordinates through perm (a random gather costs a 128-byte line per value):
+//   {t_x, t_y, meta, i}, t = i0 - xs (FP32 from the FP64 fold, i0 = ceil(xs - ns/2)),
+//   meta = ox | oy << 12 | s << 24 with (ox, oy) = i0 - (bin origin - ns/2), the support
+//   start in the bin's tile, and s the 4 x 4 sub-bin of the matrix-core geometry
+//   (bins of 4 x 4 sub-bins of sb = 16 - ns cells; 0 when sb == 0).
 struct RecArgs {
   const void* coord[2];
-  int nf[2], bs[2], nb0;
+  int nf[2], bs[2], nb0, ns, sb;
   float4* recs;
 };
 
-template <typename CT, int NS>
 __device__ __forceinline__ float4 point_record(const RecArgs& ra, int key, double x, double y, int i) {
-  constexpr int SB = 16 - NS;
   const int bx = key % ra.nb0, by = key / ra.nb0;
   const int bin_x0 = bx * ra.bs[0], bin_y0 = by * ra.bs[1];
+  const int half = ra.ns / 2;
   const Folded fx = fold_coord(x, ra.nf[0]);
   const Folded fy = fold_coord(y, ra.nf[1]);
-  const int i0x = support_start(fx, NS / 2), i0y = support_start(fy, NS / 2);
-  const int sx = (fx.cell - bin_x0) / SB, sy = (fy.cell - bin_y0) / SB;
-  const int ox = (i0x - bin_x0 + NS / 2) - sx * SB, oy = (i0y - bin_y0 + NS / 2) - sy * SB;  // in [0, SB]
+  const int i0x = support_start(fx, half), i0y = support_start(fy, half);
+  const int ox = i0x - bin_x0 + half, oy = i0y - bin_y0 + half;  // in [0, bs]
+  const int sub = ra.sb > 0 ? ((fy.cell - bin_y0) / ra.sb) * 4 + (fx.cell - bin_x0) / ra.sb : 0;
   return make_float4((float)((double)i0x - fx.xs), (float)((double)i0y - fy.xs),
-                     __int_as_float(ox | (oy << 8) | ((sy * 4 + sx) << 16)), __int_as_float(i));
+                     __int_as_float(ox | (oy << 12) | (sub << 24)), __int_as_float(i));
 }
 
 // k_scatter: W waves per chunk; wave w owns keys [lo + w*SC, lo + (w+1)*SC).
-// RNS > 0: also write the point records of ns = RNS (coordinates read in index order).
-template <typename CT, int RNS>
+// REC: also write the 2D point records (coordinates read in index order).
+template <typename CT, bool REC>
 __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ keys, int64_t M, int SC, int nbins,
                                                   int key_bits, const int32_t* __restrict__ H,
                                                   int32_t* __restrict__ perm, RecArgs ra) {
@@ -187,10 +189,10 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
       const int pos = b0 + rank;
       if (valid && pos >= 0 && (int64_t)pos < M) {
         perm[pos] = (int32_t)i;
-        if constexpr (RNS > 0) {
+        if constexpr (REC) {
           const double x = (double)static_cast<const CT*>(ra.coord[0])[i];
           const double y = (double)static_cast<const CT*>(ra.coord[1])[i];
-          ra.recs[pos] = point_record<CT, RNS>(ra, key, x, y, (int)i);
+          ra.recs[pos] = point_record(ra, key, x, y, (int)i);
         }
       }
     }
@@ -322,8 +324,7 @@ __global__ void k_empty_sort(int nbins, int32_t* bin_start, int32_t* nsub) {
 }
 
 template <typename CT, int D>
-void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int SC, const RecArgs& ra, int rec_ns,
-                        hipStream_t s) {
+void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int SC, const RecArgs& ra, hipStream_t s) {
   int key_bits = 0;
   while ((1 << key_bits) < nbins) ++key_bits;
   const int CH = W * SC;
@@ -338,18 +339,12 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
                      ps.max_subprob, ps.sub_start, ps.nsub);
   hipLaunchKernelGGL(k_row_offsets, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part, ps.bin_start);
   const size_t lds = (size_t)W * nbins * sizeof(int32_t);
-  switch (D == 2 ? rec_ns : 0) {
-    case 2: hipLaunchKernelGGL((k_scatter<CT, 2>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                               ps.hist, ps.perm, ra); break;
-    case 4: hipLaunchKernelGGL((k_scatter<CT, 4>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                               ps.hist, ps.perm, ra); break;
-    case 6: hipLaunchKernelGGL((k_scatter<CT, 6>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                               ps.hist, ps.perm, ra); break;
-    case 8: hipLaunchKernelGGL((k_scatter<CT, 8>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                               ps.hist, ps.perm, ra); break;
-    default: hipLaunchKernelGGL((k_scatter<CT, 0>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins,
-                                key_bits, ps.hist, ps.perm, ra); break;
-  }
+  if (D == 2 && ra.recs)
+    hipLaunchKernelGGL((k_scatter<CT, true>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
+                       ps.hist, ps.perm, ra);
+  else
+    hipLaunchKernelGGL((k_scatter<CT, false>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
+                       ps.hist, ps.perm, ra);
   const unsigned nfill = (unsigned)std::min<int64_t>((ps.max_subprob + 255) / 256, 4096);
   hipLaunchKernelGGL(k_subprob_fill, dim3(std::max(1u, nfill)), dim3(256), 0, s, ps.bin_start, ps.sub_start, nbins,
                      ps.maxsub, ps.nsub, ps.subprob);
@@ -358,7 +353,7 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
 }  // namespace
 
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
-                         int rec_ns) {
+                         bool want_recs, int sub_sb) {
   hipError_t e;
   const int nbins = g.nbins;
   if ((int64_t)nbins * sizeof(int32_t) > 65536) {
@@ -388,7 +383,10 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   const int64_t nparts = nseg * nbins;
   const int64_t max_sub = nbins + (M + ps.maxsub - 1) / ps.maxsub;
   if ((e = grow_buffer(ps.perm, ps.cap_M, M)) != hipSuccess) return e;
-  if (rec_ns > 0 && (e = grow_buffer(ps.recs, ps.cap_recs, M)) != hipSuccess) return e;
+  const bool recs = want_recs && g.dim == 2 && M > 0;
+  if (recs && (e = grow_buffer(ps.recs, ps.cap_recs, M)) != hipSuccess) return e;
+  ps.has_recs = false;
+  ps.recs_subbin = false;
   if ((e = grow_buffer(ps.keys, ps.cap_keys, M)) != hipSuccess) return e;
   if ((e = grow_buffer(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1)) != hipSuccess) return e;
   if ((e = grow_buffer(ps.hist, ps.cap_hist, nhist)) != hipSuccess) return e;
@@ -418,17 +416,21 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   ra.bs[0] = g.bs[0];
   ra.bs[1] = g.bs[1];
   ra.nb0 = g.nb[0];
-  ra.recs = rec_ns > 0 ? ps.recs : nullptr;
+  ra.ns = g.ns;
+  ra.sb = sub_sb;
+  ra.recs = recs ? ps.recs : nullptr;
   if (ps.coord_f64) {
-    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, ra, 0, s);
-    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, ra, rec_ns, s);
-    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, W, isc, ra, 0, s);
+    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, ra, s);
+    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, ra, s);
+    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, W, isc, ra, s);
   } else {
-    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, W, isc, ra, 0, s);
-    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, ra, rec_ns, s);
-    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, ra, 0, s);
+    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, W, isc, ra, s);
+    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, ra, s);
+    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, ra, s);
   }
-  return hipGetLastError();
+  e = hipGetLastError();
+  if (e == hipSuccess && recs) ps.has_recs = true;
+  return e;
 }
 
 void pointset_free(PointSet& ps) {
@@ -442,6 +444,7 @@ void pointset_free(PointSet& ps) {
   ps.sub_off = nullptr;
   ps.cmax = nullptr;
   ps.has_recs = false;
+  ps.recs_subbin = false;
   ps.cap_recs = ps.cap_sub_off = ps.cap_cmax = 0;
   ps.sub_start = nullptr;
   ps.hist = nullptr;

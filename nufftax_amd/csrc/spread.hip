@@ -8,16 +8,39 @@ namespace nufftx {
 
 namespace {
 
+// cmax[t] = max |re|, |im| of the strengths of transform t (FP32 fixed-point scale).
+hipError_t strength_absmax(const GridOpArgs& ga, const void* c, hipStream_t s) {
+  PointSet& ps = const_cast<PointSet&>(*ga.ps);
+  hipError_t e;
+  const int nb = (int)std::min<int64_t>((ps.M + 1023) / 1024, kAbsmaxBlocks);
+  // cmax holds n_trans floats followed by the per-block partial maxima
+  if ((e = grow_buffer(ps.cmax, ps.cap_cmax, ga.n_trans * (1 + (int64_t)nb))) != hipSuccess) return e;
+  unsigned* part = reinterpret_cast<unsigned*>(ps.cmax + ga.n_trans);
+  hipLaunchKernelGGL(k_absmax_part, dim3(nb, (unsigned)ga.n_trans), dim3(256), 0, s, static_cast<const cplx<float>*>(c),
+                     ps.M, part);
+  hipLaunchKernelGGL(k_absmax_final, dim3((unsigned)ga.n_trans), dim3(256), 0, s, part, nb, ps.cmax);
+  return hipGetLastError();
+}
+
 template <typename T, typename CT, int D>
 hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
-  const SIArgs<T> a = make_si_args<T>(ga);
   size_t lds = 0;
   const bool tiled = use_tiles(*ga.g, ga.prec, lds, true);
   const PointSet& ps = *ga.ps;
+  if constexpr (std::is_same<T, float>::value) {
+    if (tiled) {
+      hipError_t e = strength_absmax(ga, c, s);
+      if (e != hipSuccess) return e;
+    }
+  }
+  SIArgs<T> a = make_si_args<T>(ga);
+  a.cmax = ps.cmax;
   return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;
     if (tiled) {
       auto kern = k_spread_tiled<T, CT, D, NS>;
+      if constexpr (D == 2 && std::is_same<T, float>::value)
+        if (a.recs) kern = k_spread_tiled<T, CT, D, NS, true>;
       if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
                          static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
@@ -34,17 +57,13 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
 // kernel's wave tiles fit in LDS; false means "use the LDS-tiled kernel".
 bool spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hipError_t& err) {
   PointSet& ps = const_cast<PointSet&>(*ga.ps);
-  if (!ps.has_recs) return false;
+  if (!ps.recs_subbin) return false;
   const SIArgs<float> a = make_si_args<float>(ga);
   auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;
     const size_t lds = MfmaGeom<NS>::spread_lds();
     if (lds > (size_t)lds_device_max_bytes()) return false;
-    if ((err = grow_buffer(ps.cmax, ps.cap_cmax, ga.n_trans)) != hipSuccess) return true;
-    if ((err = launch_zero(ps.cmax, ga.n_trans * (int64_t)sizeof(float), s)) != hipSuccess) return true;
-    const unsigned nred = (unsigned)std::min<int64_t>((ps.M + 255) / 256, 512);
-    hipLaunchKernelGGL(k_absmax, dim3(nred, (unsigned)ga.n_trans), dim3(256), 0, s,
-                       static_cast<const cplx<float>*>(c), ps.M, ps.cmax);
+    if ((err = strength_absmax(ga, c, s)) != hipSuccess) return true;
     auto kern = k_spread_mfma2d<NS>;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kMfmaThreads), lds, s, a, ps.recs,
