@@ -27,6 +27,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes moved per launch of the dominant kernels on this workload, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch) scaled by what
+# the same counters read for a 256 MiB copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
+# the summary with the commands is profiles/nufft2d_radial_4M_hbm_traffic.md.
+TRAFFIC_BYTES = {
+    "k_spread_tiled<float,float,2,8>": 494261.3 * 1024 * 2.0 + 78768.4 * 1024,          # 1.09 GB
+    "k_gather_tiled<float,float,2,8,false,true>": 59159.2 * 1024 * 2.0 + 123500.8 * 1024,  # 0.25 GB
+}
+
 
 def _parse():
     ap = argparse.ArgumentParser()
@@ -167,12 +176,14 @@ def main():
     t_t2_res = timeit(lambda: check(L.nufftx_execute(p2.h, ptr(cout), ptr(fd)), "t2"))
     t_sort = timeit(lambda: setpts(p1))
 
-    # dominant-kernel timing with HIP events on the plan stream (grid plan, same geometry)
+    # dominant-kernel timing with HIP events on the plan stream: the spread stage of the
+    # type-1 plan and the interpolation stage of the type-2 plan (nufftx_spread_add /
+    # nufftx_interp run exactly the kernels of nufftx_execute on the plan's sorted points)
     from nufftax_amd._lib import kernel_params
     ns, beta, kc = kernel_params(eps, 2.0)
     nf = (p1.grid_size(0), p1.grid_size(1))
-    gp = grid_plan(2, nf, ns, beta, kc, 1, F32, dev)
-    gp.setpts([xd, yd])
+    setpts(p1)
+    setpts(p2)
     grid = torch.zeros((nf[1], nf[0]), dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream(dev)
     reps = 20
@@ -187,8 +198,8 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / reps * 1e-3
 
-    t_spread = ev_time(lambda: gp.spread_add(cd, grid))
-    t_interp = ev_time(lambda: gp.interp(grid, cout))
+    t_spread = ev_time(lambda: check(L.nufftx_spread_add(p1.h, ptr(cd), ptr(grid)), "spread"))
+    t_interp = ev_time(lambda: check(L.nufftx_interp(p2.h, ptr(grid), ptr(cout)), "interp"))
     nfc = nf[0] * nf[1]
     bytes_spread = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c gathered; grid written once
     bytes_interp = M * (4 + 8 + 8) + 8 * nfc   # perm, x/y, c written; grid read once
@@ -198,9 +209,10 @@ def main():
         kname = "k_spread_mfma2d<8>" if mfma else "k_spread_tiled<float,float,2,8>"
         kt, kb = t_spread, bytes_spread
     else:
-        kname = "k_interp_mfma2d<8>" if mfma else "k_gather_tiled<float,float,2,8,false>"
+        kname = "k_interp_mfma2d<8>" if mfma else "k_gather_tiled<float,float,2,8,false,true>"
         kt, kb = t_interp, bytes_interp
     achieved = kb / kt / 1e9
+    traffic = None if mfma else TRAFFIC_BYTES.get(kname)
 
     # accuracy against the FP64 direct NUDFT on subsets
     rng = np.random.default_rng(1)
@@ -237,7 +249,7 @@ def main():
         "accuracy": {"nufft2d1_rel_l2_vs_direct_64_modes": rel_l2(got1, ref1),
                      "nufft2d2_rel_l2_vs_direct_64_points": rel_l2(got2, ref2)},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": kb,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": kb,
                      "kernel_ms": kt * 1e3},
     }
     if not args.no_cpu_baseline:
