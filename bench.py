@@ -43,7 +43,8 @@ def _parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=250_000, help="points in the CPU-baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000, help="points in the CPU-baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=7, help="timed CPU-baseline runs (median reported)")
     ap.add_argument("--profile-only", action="store_true", help="run a few steps, print nothing but a status line")
     return ap.parse_args()
 
@@ -76,7 +77,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def cpu_baseline(x, y, c, f, nm, eps, sample):
+def cpu_baseline(x, y, c, f, nm, eps, sample, runs):
     """The oracle (NumPy restatement of the reference's pure-JAX algorithm, complex64 like
     the reference) on the first `sample` points of the same workload, 1 thread."""
     from oracle import nufft_ref as R
@@ -84,7 +85,7 @@ def cpu_baseline(x, y, c, f, nm, eps, sample):
     xs, ys, cs = x[:sample], y[:sample], c[:sample]
     R.nufft_type1([xs[:1000], ys[:1000]], cs[:1000], nm, eps=eps)  # warm
     times = []
-    for _ in range(2):
+    for _ in range(max(1, runs)):
         t0 = time.perf_counter()
         R.nufft_type1([xs, ys], cs, nm, eps=eps, isign=1)
         R.nufft_type2([xs, ys], f, eps=eps, isign=-1)
@@ -253,7 +254,7 @@ def main():
                      "kernel_ms": kt * 1e3},
     }
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(x32, y32, c64, f64, nm, eps, args.cpu_sample)
+        res["cpu_baseline"] = cpu_baseline(x32, y32, c64, f64, nm, eps, args.cpu_sample, args.cpu_runs)
     print(json.dumps(res), flush=True)
 
 

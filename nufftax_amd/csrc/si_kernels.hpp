@@ -12,6 +12,7 @@
 // Global path (tile larger than the LDS budget): same per-point work directly
 // on the fine grid in HBM, points visited in sorted order.
 #pragma once
+#include <cstdlib>
 
 #include <type_traits>
 
@@ -584,10 +585,17 @@ SIArgs<T> make_si_args(const GridOpArgs& ga) {
 
 // LDS bytes per tile cell: the spread tile holds two 8-byte accumulators per cell
 // (64-bit fixed point or FP64), the gather tile one complex value.
+// Tiles up to the device's per-block LDS limit (160 KB on gfx950) run tiled; bins are
+// sized for the smaller occupancy budget first, but the bin-count cap of the sort can
+// force larger tiles (3D grids of 256^3 cells at ns = 7: 23 x 15 x 15 cells, 83 KB).
+// NUFFTX_LDS_TILE_MAX lowers the limit (read per call), which sends larger tiles to
+// the global-memory kernels.
 inline bool use_tiles(const Geometry& g, int prec, size_t& lds, bool spread = false) {
   const size_t cell = spread ? 16 : (prec == kF32 ? 8 : 16);
   lds = (size_t)g.tile_cells * cell;
-  return lds <= (size_t)lds_budget_bytes();
+  size_t cap = (size_t)lds_device_max_bytes();
+  if (const char* e = std::getenv("NUFFTX_LDS_TILE_MAX")) cap = std::min(cap, (size_t)std::max(4096, std::atoi(e)));
+  return lds <= cap;
 }
 
 }  // namespace nufftx

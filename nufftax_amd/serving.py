@@ -363,13 +363,33 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
     return latency_stats(records)
 
 
+def service_times(reqs) -> dict:
+    """Each request alone on an idle GPU (synchronised before and after): the service-time
+    distribution in ms, overall and per (type, dimension, n_modes) class."""
+    dev = reqs[0][1][0].device
+    per_class = {}
+    all_ms = []
+    for kind, coords, data, nm in reqs:
+        torch.cuda.synchronize(dev)
+        t = time.perf_counter()
+        _direct_call(kind, coords, data, nm)
+        torch.cuda.synchronize(dev)
+        ms = (time.perf_counter() - t) * 1e3
+        all_ms.append(ms)
+        per_class.setdefault(f"type{kind} {len(coords)}D {nm[0]}^{len(nm)}", []).append(ms)
+    a = np.array(all_ms)
+    return {"n": len(a), "p50_ms": float(np.percentile(a, 50)), "p99_ms": float(np.percentile(a, 99)),
+            "mean_ms": float(a.mean()),
+            "per_class_mean_ms": {k: round(float(np.mean(v)), 4) for k, v in sorted(per_class.items())}}
+
+
 def main(argv=None):
     import argparse
     import json
 
     ap = argparse.ArgumentParser(description="p50/p99 latency of the mixed NUFFT serving stream")
     ap.add_argument("--requests", type=int, default=200)
-    ap.add_argument("--qps", type=float, default=100.0)
+    ap.add_argument("--qps", type=float, nargs="+", default=[100.0], help="one stream replay per rate")
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
@@ -383,11 +403,14 @@ def main(argv=None):
             seen.add(key)
             _direct_call(kind, coords, data, nm)
     torch.cuda.synchronize(dev)
-    res = {"workload": "mixed 2D/3D type 1/2 stream, complex64, eps 1e-6", "requests": args.requests,
-           "qps": args.qps, "streams": args.streams,
-           "server": run_stream(reqs, args.qps, args.seed, "server", args.streams),
-           "per_call": run_stream(reqs, args.qps, args.seed, "per-call")}
-    print(json.dumps(res), flush=True)
+    svc = service_times(reqs)
+    print(json.dumps({"workload": "mixed 2D/3D type 1/2 stream, complex64, eps 1e-6",
+                      "requests": args.requests, "service": svc}), flush=True)
+    for qps in args.qps:
+        res = {"qps": qps, "streams": args.streams,
+               "server": run_stream(reqs, qps, args.seed, "server", args.streams),
+               "per_call": run_stream(reqs, qps, args.seed, "per-call")}
+        print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

@@ -67,6 +67,47 @@ def test_bin_sort_bit_exact(rng, dim, n_modes, M, dist):
     np.testing.assert_array_equal(_np(bst).astype(np.int64), ref_off)
 
 
+def _check_sort(coords, n_modes, kind=1):
+    """perm and bin offsets of a plan's sort equal the oracle's stable sort of the same keys."""
+    from nufftax_amd._plan import F32, transform_plan
+
+    dim = len(coords)
+    xs = [_t(c.astype(np.float32)) for c in coords]
+    plan = transform_plan(kind, dim, n_modes, 1, 1e-6, 1 if kind == 1 else -1, 2.0, F32, torch.device(DEV))
+    plan.setpts(xs)
+    perm, bst, nbins, bs, nbd = plan.sort_state()
+    torch.cuda.synchronize()
+    nfs = [plan.grid_size(d) for d in range(dim)]
+    keys = R.bin_keys([c.astype(np.float32).astype(np.float64) for c in coords], nfs, bs[:dim])
+    ref_perm, ref_off = R.bin_sort(keys, nbins)
+    np.testing.assert_array_equal(_np(perm).astype(np.int64), ref_perm)
+    np.testing.assert_array_equal(_np(bst).astype(np.int64), ref_off)
+
+
+@pytest.mark.parametrize("M", [1, 63, 64, 65, 8191, 8193, 16384 + 5])
+@pytest.mark.parametrize("kind", [1, 2])
+def test_bin_sort_ragged_sizes(rng, M, kind):
+    """sizes around the 64-key ranking step and the 8192-point chunk; type 2 also writes records."""
+    _check_sort([rng.uniform(-np.pi, np.pi, M) for _ in range(2)], (64, 48), kind)
+
+
+def test_bin_sort_one_bin_many_points(rng):
+    """every point in one bin (the one-comparison ranking path), plus a second bin at the end."""
+    M = 100_000
+    x = np.full(M, 0.25)
+    y = np.full(M, -1.0)
+    x[-3:] = 2.5
+    _check_sort([x, y], (128, 128), kind=2)
+
+
+def test_bin_sort_radial_4m_full_size():
+    """the benchmark's point set (3,999,744 radial samples, 1024^2 grid), type-2 plan (records)."""
+    from nufftax_amd import workloads
+
+    x, y, _, _ = workloads.radial_2d(seed=0)
+    _check_sort([x, y], (512, 512), kind=2)
+
+
 # ------------------------------------------------------------------ spread / interp
 
 
@@ -101,7 +142,8 @@ def test_spread_global_path_matches_tiled(rng, monkeypatch):
     """A tile larger than the LDS budget takes the global-memory path: same result."""
     from nufftax_amd._plan import F64, grid_plan
 
-    kp = R.compute_kernel_params(1e-14)  # ns = 16 in 3D FP64: tile does not fit LDS
+    monkeypatch.setenv("NUFFTX_LDS_TILE_MAX", "65536")  # ns = 16 in 3D FP64: 128 KB tiles
+    kp = R.compute_kernel_params(1e-14)
     nfs = (40, 36, 34)
     M = 3000
     coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(3)]
