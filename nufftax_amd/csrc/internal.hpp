@@ -1,5 +1,6 @@
 // Internal interfaces between the nufftx translation units (host side).
 #pragma once
+#include <vector>
 
 #include <hip/hip_runtime.h>
 
@@ -54,6 +55,7 @@ struct PointSet {
   float* cmax = nullptr;          // n_trans: max |re|, |im| of the strengths (matrix-core spread)
   int64_t cap_recs = 0, cap_sub_off = 0, cap_cmax = 0;
   int64_t cap_M = 0, cap_keys = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0, cap_sub_start = 0;
+  std::vector<void*> retired;    // outgrown buffers, freed by pointset_free
 };
 
 struct SortTuning {
@@ -73,22 +75,22 @@ int lds_device_max_bytes();
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
                          bool want_recs = false, int sub_sb = 0);
 
-// device buffer of at least `need` elements (contents not kept when it grows)
+// Device buffer of at least `need` elements (contents not kept when it grows).  The
+// outgrown buffer is not freed here: hipFree waits for the whole device, which would
+// stall every other stream (a serving loop runs one plan per stream).  It is moved to
+// `retired` and freed when its owner is destroyed; capacity grows by at least half
+// each time, so the retired buffers stay below twice the live one.  On failure the old
+// buffer and capacity are kept.
 template <typename T>
-hipError_t grow_buffer(T*& p, int64_t& cap, int64_t need) {
+hipError_t grow_buffer(T*& p, int64_t& cap, int64_t need, std::vector<void*>& retired) {
   if (need <= cap && p) return hipSuccess;
-  if (p) {
-    hipError_t e = hipFree(p);
-    if (e != hipSuccess) return e;
-    p = nullptr;
-  }
-  const int64_t n = need > 1 ? need : 1;
-  hipError_t e = hipMalloc((void**)&p, (size_t)n * sizeof(T));
-  if (e != hipSuccess) {
-    cap = 0;
-    p = nullptr;
-    return e;
-  }
+  int64_t n = need > 1 ? need : 1;
+  if (p && cap + cap / 2 > n) n = cap + cap / 2;
+  T* q = nullptr;
+  hipError_t e = hipMalloc((void**)&q, (size_t)n * sizeof(T));
+  if (e != hipSuccess) return e;
+  if (p) retired.push_back(p);
+  p = q;
   cap = n;
   return hipSuccess;
 }

@@ -255,17 +255,13 @@ int dev_alloc(T*& p, size_t n) {
   return NUFFTX_OK;
 }
 
+// Grow a plan-owned buffer without freeing the outgrown one (see grow_buffer).
 template <typename T>
-int dev_grow(T*& p, int64_t& cap, int64_t n) {
+int dev_grow(T*& p, int64_t& cap, int64_t n, std::vector<void*>& retired) {
   if (p && n <= cap) return NUFFTX_OK;
-  if (p) {
-    (void)hipFree(p);
-    p = nullptr;
-  }
-  cap = 0;
-  int r = dev_alloc(p, (size_t)std::max<int64_t>(n, 1));
-  if (r == NUFFTX_OK) cap = std::max<int64_t>(n, 1);
-  return r;
+  if (grow_buffer(p, cap, n, retired) != hipSuccess)
+    return fail(NUFFTX_ERR_ALLOC, "hipMalloc failed (" + std::to_string(n * sizeof(T)) + " bytes)");
+  return NUFFTX_OK;
 }
 
 int upload_factors(void*& dst, const std::vector<double>& f, int prec) {
@@ -322,6 +318,7 @@ struct nufftx_plan_s {
   double* post = nullptr;
   void* cbuf = nullptr;
   int64_t cap_xr[3] = {0, 0, 0}, cap_sr[3] = {0, 0, 0}, cap_pre = 0, cap_post = 0, cap_cbuf = 0;
+  std::vector<void*> retired;  // outgrown type-3 buffers, freed with the plan
   int64_t M = 0, N = 0;
   size_t csize() const { return prec == kF32 ? 8 : 16; }
 };
@@ -338,6 +335,8 @@ void plan_free(nufftx_plan p) {
                   p->cbuf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  for (void* b : p->retired) (void)hipFree(b);
+  p->retired.clear();
 }
 
 int check_plan(nufftx_plan p) {
@@ -594,14 +593,14 @@ int nufftx_setpts(nufftx_plan p, int64_t M, const void* x, const void* y, const 
   p->N = N;
   int r;
   for (int d = 0; d < p->dim; ++d) {
-    if ((r = dev_grow(p->xr[d], p->cap_xr[d], M))) return r;
-    if ((r = dev_grow(p->sr[d], p->cap_sr[d], N))) return r;
+    if ((r = dev_grow(p->xr[d], p->cap_xr[d], M, p->retired))) return r;
+    if ((r = dev_grow(p->sr[d], p->cap_sr[d], N, p->retired))) return r;
   }
-  if ((r = dev_grow(p->pre, p->cap_pre, 2 * M))) return r;
-  if ((r = dev_grow(p->post, p->cap_post, 2 * N))) return r;
+  if ((r = dev_grow(p->pre, p->cap_pre, 2 * M, p->retired))) return r;
+  if ((r = dev_grow(p->post, p->cap_post, 2 * N, p->retired))) return r;
   {
     char* cb = static_cast<char*>(p->cbuf);
-    if ((r = dev_grow(cb, p->cap_cbuf, M * p->n_trans * (int64_t)p->csize()))) return r;
+    if ((r = dev_grow(cb, p->cap_cbuf, M * p->n_trans * (int64_t)p->csize(), p->retired))) return r;
     p->cbuf = cb;
   }
   const void* const xa[3] = {x, p->dim > 1 ? y : x, p->dim > 2 ? z : x};

@@ -121,7 +121,7 @@ hipError_t pointset_subsort(PointSet& ps, const Geometry& g, hipStream_t s) {
   ps.recs_subbin = false;
   if (ps.M == 0 || !ps.has_recs) return hipSuccess;
   hipError_t e;
-  if ((e = grow_buffer(ps.sub_off, ps.cap_sub_off, (int64_t)ps.max_subprob * 17)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.sub_off, ps.cap_sub_off, (int64_t)ps.max_subprob * 17, ps.retired)) != hipSuccess) return e;
   PrepArgs a;
   a.subprob = ps.subprob;
   a.nsub = ps.nsub;

@@ -382,17 +382,17 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   const int64_t nseg = (NB + kRowSeg - 1) / kRowSeg;
   const int64_t nparts = nseg * nbins;
   const int64_t max_sub = nbins + (M + ps.maxsub - 1) / ps.maxsub;
-  if ((e = grow_buffer(ps.perm, ps.cap_M, M)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.perm, ps.cap_M, M, ps.retired)) != hipSuccess) return e;
   const bool recs = want_recs && g.dim == 2 && M > 0;
-  if (recs && (e = grow_buffer(ps.recs, ps.cap_recs, M)) != hipSuccess) return e;
+  if (recs && (e = grow_buffer(ps.recs, ps.cap_recs, M, ps.retired)) != hipSuccess) return e;
   ps.has_recs = false;
   ps.recs_subbin = false;
-  if ((e = grow_buffer(ps.keys, ps.cap_keys, M)) != hipSuccess) return e;
-  if ((e = grow_buffer(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1)) != hipSuccess) return e;
-  if ((e = grow_buffer(ps.hist, ps.cap_hist, nhist)) != hipSuccess) return e;
-  if ((e = grow_buffer(ps.scan_part, ps.cap_part, nparts)) != hipSuccess) return e;
-  if ((e = grow_buffer(ps.bin_start, ps.cap_bins, (int64_t)nbins + 1)) != hipSuccess) return e;
-  if ((e = grow_buffer(ps.subprob, ps.cap_sub, max_sub)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.keys, ps.cap_keys, M, ps.retired)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1, ps.retired)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.hist, ps.cap_hist, nhist, ps.retired)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.scan_part, ps.cap_part, nparts, ps.retired)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.bin_start, ps.cap_bins, (int64_t)nbins + 1, ps.retired)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.subprob, ps.cap_sub, max_sub, ps.retired)) != hipSuccess) return e;
   if (!ps.nsub && (e = hipMalloc((void**)&ps.nsub, sizeof(int32_t))) != hipSuccess) return e;
   ps.max_subprob = (int)max_sub;
 
@@ -438,6 +438,8 @@ void pointset_free(PointSet& ps) {
                   ps.recs, ps.sub_off, ps.cmax};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  for (void* p : ps.retired) (void)hipFree(p);
+  ps.retired.clear();
   ps.perm = nullptr;
   ps.keys = nullptr;
   ps.recs = nullptr;

@@ -14,7 +14,7 @@ hipError_t strength_absmax(const GridOpArgs& ga, const void* c, hipStream_t s) {
   hipError_t e;
   const int nb = (int)std::min<int64_t>((ps.M + 1023) / 1024, kAbsmaxBlocks);
   // cmax holds n_trans floats followed by the per-block partial maxima
-  if ((e = grow_buffer(ps.cmax, ps.cap_cmax, ga.n_trans * (1 + (int64_t)nb))) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.cmax, ps.cap_cmax, ga.n_trans * (1 + (int64_t)nb), ps.retired)) != hipSuccess) return e;
   unsigned* part = reinterpret_cast<unsigned*>(ps.cmax + ga.n_trans);
   hipLaunchKernelGGL(k_absmax_part, dim3(nb, (unsigned)ga.n_trans), dim3(256), 0, s, static_cast<const cplx<float>*>(c),
                      ps.M, part);

@@ -53,7 +53,8 @@ class ServeRequest:
     stream_index: int | None = None  # pin to one of the server's streams (plan warm-up)
     nbytes: int = 0
     t_submit: float = 0.0
-    t_start: float = 0.0
+    t_start: float = 0.0         # taken by the dispatcher
+    t_issued: float = 0.0        # all of its launches queued on the stream
     t_done: float = 0.0
 
     @property
@@ -209,8 +210,9 @@ class NufftServer:
                 for g in group:
                     g.future.set_exception(e)
                 continue
+            t1 = time.perf_counter()
             for g in group:
-                g.t_start = t0
+                g.t_start, g.t_issued = t0, t1
             self._done_q.put((ev, group, outs, nbytes))
 
     def _launch(self, group):
@@ -264,11 +266,17 @@ class NufftServer:
 
 
 def latency_stats(requests) -> dict:
-    """p50 / p99 / mean latency (submit -> done) in ms, and requests per second over the span."""
+    """p50 / p99 / mean latency (submit -> done) in ms, requests per second over the span, and
+    the mean of each phase: queued (submit -> taken), issue (taken -> launches queued) and
+    device (launches queued -> completion seen on the host)."""
     lat = np.array([(r.t_done - r.t_submit) * 1e3 for r in requests])
     span = max(r.t_done for r in requests) - min(r.t_submit for r in requests)
+    mean = lambda v: float(np.mean(v)) * 1e3  # noqa: E731
     return {"n": len(lat), "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "throughput_rps": len(lat) / span if span > 0 else float("inf")}
+            "mean_ms": float(lat.mean()), "throughput_rps": len(lat) / span if span > 0 else float("inf"),
+            "phase_mean_ms": {"queued": mean([r.t_start - r.t_submit for r in requests]),
+                              "issue": mean([r.t_issued - r.t_start for r in requests]),
+                              "device": mean([r.t_done - r.t_issued for r in requests])}}
 
 
 # ---------------------------------------------------------------------- stream harness
@@ -315,11 +323,14 @@ def _direct_call(kind, coords, data, nm, eps=1e-6):
 
 
 def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4):
-    """Replay reqs with Poisson arrivals at qps.  mode "server": NufftServer; mode "per-call":
-    the same transforms called one at a time (each synchronised) in arrival order."""
+    """Replay reqs with Poisson arrivals at qps (qps <= 0: every request arrives at once, which
+    measures the saturated throughput).  mode "server": NufftServer; mode "per-call": the same
+    transforms called one at a time (each synchronised) in arrival order."""
     rng = np.random.default_rng(seed + 1)
-    gaps = rng.exponential(1.0 / qps, len(reqs))
-    arrivals = np.cumsum(gaps)
+    if qps > 0:
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, len(reqs)))
+    else:
+        arrivals = np.zeros(len(reqs))
     dev = reqs[0][1][0].device
     torch.cuda.synchronize(dev)
     records = []
@@ -357,6 +368,7 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
             r.t_submit = t0 + ta
             r.t_start = time.perf_counter()
             _direct_call(kind, coords, data, nm)
+            r.t_issued = time.perf_counter()
             torch.cuda.synchronize(dev)
             r.t_done = time.perf_counter()
             records.append(r)
@@ -389,7 +401,8 @@ def main(argv=None):
 
     ap = argparse.ArgumentParser(description="p50/p99 latency of the mixed NUFFT serving stream")
     ap.add_argument("--requests", type=int, default=200)
-    ap.add_argument("--qps", type=float, nargs="+", default=[100.0], help="one stream replay per rate")
+    ap.add_argument("--qps", type=float, nargs="+", default=[100.0],
+                    help="one stream replay per rate; 0 = all requests at once (saturated throughput)")
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
