@@ -42,6 +42,8 @@ struct PointSet {
   int32_t* keys = nullptr;       // M, bin key of every point (index order)
   int32_t* hist = nullptr;       // nbins * NB (+ scan scratch)
   int32_t* scan_part = nullptr;  // scan partials
+  int32_t* scan_sum = nullptr;   // segment sums of the histogram (zero between sorts)
+  int64_t cap_sum = 0;
   int32_t* bin_start = nullptr;  // nbins + 1
   int32_t* sub_start = nullptr;  // nbins + 1, first subproblem of every bin
   int4* subprob = nullptr;       // max_subprob

@@ -4,21 +4,22 @@
 // Contract: perm == argsort(key, stable) and bin_start == [0, cumsum(bincount(key))],
 // where key = sum_d (floor(xs_d) div bs_d) * prod_{e<d} nb_e and xs is the FP64 fold.
 //
-// Counting sort, all integer work:
+// Counting sort, all integer work, in three launches:
 //   A  k_keys: one block per chunk of points folds each point once, stores its
 //      key and counts the keys in LDS (per-lane integer atomics); the block writes
-//      its row of the histogram H[chunk][bin] (coalesced).
-//   B  segmented column scan of H (k_colsum_seg, k_seg_prefix, k_bin_scan,
-//      k_row_offsets): H[chunk][bin] becomes the first output slot of
-//      (bin, chunk); bin_start the first slot of every bin; sub_start the first
-//      subproblem of every bin.
-//   C  k_scatter: the W waves of a chunk block count their share of the chunk's
-//      keys, turn the counts into per-wave bases, then walk their keys in index
-//      order, ranking equal keys inside each 64-key step (one comparison when the
-//      step's keys agree, else one ballot per key bit), and write perm[slot] = index.
-//   D  k_subprob_fill: descriptors {bin, first sorted slot, count} of the
-//      subproblems (bins split into <= maxsub points) used by the tiled
-//      spread/interp kernels, one thread per descriptor.
+//      its row of the histogram H[chunk][bin] (coalesced) and adds the row into
+//      the sums of its segment of 16 rows, S[seg][bin] (global integer atomics).
+//   B  k_seg_scan (one block): S -> exclusive prefixes over the segments (P) and
+//      bin totals, the totals -> bin_start and, per bin, the first subproblem
+//      (sub_start); S is cleared for the next sort.
+//   C  k_scatter: each block first fills its share of the subproblem descriptors
+//      {bin, first sorted slot, count} (bins split into <= maxsub points) used by
+//      the tiled spread/interp kernels; then the W waves of a chunk count their
+//      share of the chunk's keys, turn the counts into per-wave bases (bin start +
+//      segment prefix + rows of the segment before the chunk), and walk their keys
+//      in index order, ranking equal keys inside each 64-key step (one comparison
+//      when the step's keys agree, else one ballot per key bit), and write
+//      perm[slot] = index.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -73,11 +74,89 @@ __device__ __forceinline__ uint64_t match_key(int key, bool valid, int key_bits)
 constexpr int kStep = 64;          // points per wave step
 constexpr int kGroup = 8;          // steps whose coordinates are loaded together
 
+// ------------------------------------------------------------------ scan helpers
+constexpr int kRowSeg = 16;  // histogram rows per scan segment
+
+// Block-wide exclusive scan of one value per thread (blockDim.x a multiple of 64,
+// at most 1024); returns the exclusive prefix and sets the block total.
+__device__ __forceinline__ int block_excl_scan(int v, int& total) {
+  __shared__ int wsum[16];
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int t = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int wbase = 0, tot = 0;
+  for (int k = 0; k < nw; ++k) {
+    const int s = wsum[k];
+    if (k < w) wbase += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return wbase + incl - v;
+}
+
+struct ScanArgs {
+  int32_t* H;          // [NB][nbins] chunk histograms
+  int32_t* S;          // [nseg][nbins] segment sums (zero between sorts)
+  int32_t* P;          // [nseg][nbins] exclusive prefixes over the segments
+  int32_t* bin_start;  // [nbins + 1]
+  int32_t* sub_start;  // [nbins + 1]
+  int32_t* nsub;
+  int4* subprob;
+  int NB, nseg, nbins, M, maxsub, max_subprob;
+};
+
+// one block: S -> P (exclusive over segments, S cleared); bin totals -> bin_start;
+// subproblems per bin -> sub_start; bin_start[nbins] = M; nsub.
+__global__ void __launch_bounds__(1024) k_seg_scan(ScanArgs sa) {
+  for (int b = threadIdx.x; b < sa.nbins; b += blockDim.x) {
+    int run = 0;
+#pragma unroll 8
+    for (int g = 0; g < sa.nseg; ++g) {
+      const int64_t i = (int64_t)g * sa.nbins + b;
+      const int t = sa.S[i];
+      sa.S[i] = 0;
+      sa.P[i] = run;
+      run += t;
+    }
+    sa.bin_start[b] = run;
+  }
+  __syncthreads();
+  int carry = 0, carry_sub = 0;
+  for (int base = 0; base < sa.nbins; base += blockDim.x) {
+    const int b = base + threadIdx.x;
+    const int cnt = b < sa.nbins ? sa.bin_start[b] : 0;
+    const int nsb = (cnt + sa.maxsub - 1) / sa.maxsub;
+    int total, total_sub;
+    const int ex = block_excl_scan(cnt, total) + carry;
+    const int ex_sub = block_excl_scan(nsb, total_sub) + carry_sub;
+    if (b < sa.nbins) {
+      sa.bin_start[b] = ex;
+      sa.sub_start[b] = ex_sub;
+    }
+    carry += total;
+    carry_sub += total_sub;
+  }
+  if (threadIdx.x == 0) {
+    sa.bin_start[sa.nbins] = sa.M;
+    sa.sub_start[sa.nbins] = carry_sub;
+    *sa.nsub = min(carry_sub, sa.max_subprob);
+  }
+}
+
 // k_keys: a block of 64 W threads folds the points of chunk blockIdx.x
 // ([lo, lo + CH)), stores their keys and counts them.
 template <typename CT, int D>
 __global__ void __launch_bounds__(1024) k_keys(KeyArgs ka, int64_t M, int CH, int nbins, int32_t* __restrict__ keys,
-                                               int32_t* __restrict__ H) {
+                                               ScanArgs sa) {
+  int32_t* __restrict__ H = sa.H;
   extern __shared__ int32_t cnt[];
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
   __syncthreads();
@@ -103,7 +182,12 @@ __global__ void __launch_bounds__(1024) k_keys(KeyArgs ka, int64_t M, int CH, in
   }
   __syncthreads();
   int32_t* row = H + (int64_t)blockIdx.x * nbins;
-  for (int b = threadIdx.x; b < nbins; b += blockDim.x) row[b] = cnt[b];
+  int32_t* seg = sa.S + (int64_t)(blockIdx.x / kRowSeg) * nbins;
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+    const int v = cnt[b];
+    row[b] = v;
+    if (v) atomicAdd(&seg[b], v);
+  }
 }
 
 // Point records of 2D FP32 point sets, written at each point's sorted position so
@@ -136,9 +220,25 @@ __device__ __forceinline__ float4 point_record(const RecArgs& ra, int key, doubl
 // REC: also write the 2D point records (coordinates read in index order).
 template <typename CT, bool REC>
 __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ keys, int64_t M, int SC, int nbins,
-                                                  int key_bits, const int32_t* __restrict__ H,
-                                                  int32_t* __restrict__ perm, RecArgs ra) {
+                                                  int key_bits, ScanArgs sa, int32_t* __restrict__ perm, RecArgs ra) {
   extern __shared__ int32_t wcnt[];  // [W][nbins]
+  // subproblem descriptors; slot's bin is the last b with sub_start[b] <= slot
+  {
+    const int n = min(*sa.nsub, sa.max_subprob);
+    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+      int lo = 0, hi = nbins;  // invariant: sub_start[lo] <= slot < sub_start[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sa.sub_start[mid] <= slot)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const int k = slot - sa.sub_start[lo];
+      const int st = sa.bin_start[lo], cnt = sa.bin_start[lo + 1] - st;
+      sa.subprob[slot] = make_int4(lo, st + k * sa.maxsub, min(sa.maxsub, cnt - k * sa.maxsub), 0);
+    }
+  }
   const int W = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int32_t* my = wcnt + wave * nbins;
@@ -158,10 +258,12 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
       if (k[u] >= 0) atomicAdd(&my[k[u]], 1);
   }
   __syncthreads();
-  // per-wave bases: chunk slot of the bin, then earlier waves' counts
-  const int32_t* row = H + (int64_t)blockIdx.x * nbins;
+  // per-wave bases: first slot of (bin, chunk) = bin start + prefix of the earlier
+  // segments + the rows of this segment before the chunk; then earlier waves' counts
+  const int chunk = (int)blockIdx.x, seg = chunk / kRowSeg;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-    int run = row[b];
+    int run = sa.bin_start[b] + sa.P[(int64_t)seg * nbins + b];
+    for (int r = seg * kRowSeg; r < chunk; ++r) run += sa.H[(int64_t)r * nbins + b];
     for (int w = 0; w < W; ++w) {
       const int t = wcnt[w * nbins + b];
       wcnt[w * nbins + b] = run;
@@ -199,123 +301,8 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
   }
 }
 
-// ------------------------------------------------------------------ scan
-// Block-wide exclusive scan of one value per thread; returns the block total.
-template <int NT>
-__device__ __forceinline__ int block_excl_scan(int v, int& total) {
-  __shared__ int wsum[NT / 64];
-  const int lane = lane_id();
-  const int w = threadIdx.x >> 6;
-  int incl = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    int t = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += t;
-  }
-  if (lane == 63) wsum[w] = incl;
-  __syncthreads();
-  int wbase = 0;
-  int tot = 0;
-#pragma unroll
-  for (int k = 0; k < NT / 64; ++k) {
-    int s = wsum[k];
-    if (k < w) wbase += s;
-    tot += s;
-  }
-  __syncthreads();
-  total = tot;
-  return wbase + incl - v;
-}
-
-constexpr int kRowSeg = 16;  // histogram rows per scan segment
-
-// P[seg][b] = sum of H[r][b] over the rows r of segment seg
-__global__ void __launch_bounds__(256) k_colsum_seg(const int32_t* __restrict__ H, int NB, int nbins,
-                                                    int32_t* __restrict__ P) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int seg = blockIdx.y;
-  if (b >= nbins) return;
-  const int r0 = seg * kRowSeg, r1 = min(NB, r0 + kRowSeg);
-  int s = 0;
-  for (int r = r0; r < r1; ++r) s += H[(int64_t)r * nbins + b];
-  P[(int64_t)seg * nbins + b] = s;
-}
-
-// P[seg][b] -> exclusive prefix over segments (in place); tot[b] = bin total.
-__global__ void __launch_bounds__(256) k_seg_prefix(int32_t* __restrict__ P, int nseg, int nbins,
-                                                    int32_t* __restrict__ tot) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  int run = 0;
-  for (int g = 0; g < nseg; ++g) {
-    const int t = P[(int64_t)g * nbins + b];
-    P[(int64_t)g * nbins + b] = run;
-    run += t;
-  }
-  tot[b] = run;
-}
-
-// single block: bin totals (in bin_start) -> exclusive scan = bin_start; subproblems
-// per bin -> exclusive scan = sub_start; bin_start[nbins] = M; nsub.
-__global__ void __launch_bounds__(1024) k_bin_scan(int32_t* __restrict__ bin_start, int nbins, int M, int maxsub,
-                                                   int max_subprob, int32_t* __restrict__ sub_start,
-                                                   int32_t* __restrict__ nsub) {
-  int carry = 0, carry_sub = 0;
-  for (int base = 0; base < nbins; base += 1024) {
-    const int b = base + threadIdx.x;
-    const int cnt = b < nbins ? bin_start[b] : 0;
-    const int nsb = (cnt + maxsub - 1) / maxsub;
-    int total, total_sub;
-    const int ex = block_excl_scan<1024>(cnt, total) + carry;
-    const int ex_sub = block_excl_scan<1024>(nsb, total_sub) + carry_sub;
-    if (b < nbins) {
-      bin_start[b] = ex;
-      sub_start[b] = ex_sub;
-    }
-    carry += total;
-    carry_sub += total_sub;
-  }
-  if (threadIdx.x == 0) {
-    bin_start[nbins] = M;
-    sub_start[nbins] = carry_sub;
-    *nsub = min(carry_sub, max_subprob);
-  }
-}
-
-// H[r][b] -> first output slot of (bin b, chunk r)
-__global__ void __launch_bounds__(256) k_row_offsets(int32_t* __restrict__ H, int NB, int nbins,
-                                                     const int32_t* __restrict__ P,
-                                                     const int32_t* __restrict__ bin_start) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  const int seg = blockIdx.y;
-  if (b >= nbins) return;
-  const int r0 = seg * kRowSeg, r1 = min(NB, r0 + kRowSeg);
-  int run = bin_start[b] + P[(int64_t)seg * nbins + b];
-  for (int r = r0; r < r1; ++r) {
-    const int t = H[(int64_t)r * nbins + b];
-    H[(int64_t)r * nbins + b] = run;
-    run += t;
-  }
-}
-
-// one thread per descriptor slot: its bin is the last b with sub_start[b] <= slot
-__global__ void __launch_bounds__(256) k_subprob_fill(const int32_t* __restrict__ bin_start,
-                                                      const int32_t* __restrict__ sub_start, int nbins, int maxsub,
-                                                      const int32_t* __restrict__ nsub, int4* __restrict__ subprob) {
-  const int n = *nsub;
-  for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
-    int lo = 0, hi = nbins;  // invariant: sub_start[lo] <= slot < sub_start[hi]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (sub_start[mid] <= slot)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const int k = slot - sub_start[lo];
-    const int st = bin_start[lo], cnt = bin_start[lo + 1] - st;
-    subprob[slot] = make_int4(lo, st + k * maxsub, min(maxsub, cnt - k * maxsub), 0);
-  }
+__global__ void k_zero_i32(int32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
 __global__ void k_empty_sort(int nbins, int32_t* bin_start, int32_t* nsub) {
@@ -328,26 +315,30 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
   int key_bits = 0;
   while ((1 << key_bits) < nbins) ++key_bits;
   const int CH = W * SC;
-  hipLaunchKernelGGL((k_keys<CT, D>), dim3(ps.NB), dim3(64 * W), (size_t)nbins * sizeof(int32_t), s, ka, ps.M, CH,
-                     nbins, ps.keys, ps.hist);
-  const int nseg = (ps.NB + kRowSeg - 1) / kRowSeg;
-  const dim3 g2((nbins + 255) / 256, nseg);
-  hipLaunchKernelGGL(k_colsum_seg, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part);
-  hipLaunchKernelGGL(k_seg_prefix, dim3((nbins + 255) / 256), dim3(256), 0, s, ps.scan_part, nseg, nbins,
-                     ps.bin_start);
-  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, ps.bin_start, nbins, (int)ps.M, ps.maxsub,
-                     ps.max_subprob, ps.sub_start, ps.nsub);
-  hipLaunchKernelGGL(k_row_offsets, g2, dim3(256), 0, s, ps.hist, ps.NB, nbins, ps.scan_part, ps.bin_start);
+  ScanArgs sa;
+  sa.H = ps.hist;
+  sa.S = ps.scan_sum;
+  sa.P = ps.scan_part;
+  sa.bin_start = ps.bin_start;
+  sa.sub_start = ps.sub_start;
+  sa.nsub = ps.nsub;
+  sa.subprob = ps.subprob;
+  sa.NB = ps.NB;
+  sa.nseg = (ps.NB + kRowSeg - 1) / kRowSeg;
+  sa.nbins = nbins;
+  sa.M = (int)ps.M;
+  sa.maxsub = ps.maxsub;
+  sa.max_subprob = ps.max_subprob;
+  const size_t lds_keys = (size_t)nbins * sizeof(int32_t);
+  hipLaunchKernelGGL((k_keys<CT, D>), dim3(ps.NB), dim3(64 * W), lds_keys, s, ka, ps.M, CH, nbins, ps.keys, sa);
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sa);
   const size_t lds = (size_t)W * nbins * sizeof(int32_t);
   if (D == 2 && ra.recs)
     hipLaunchKernelGGL((k_scatter<CT, true>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                       ps.hist, ps.perm, ra);
+                       sa, ps.perm, ra);
   else
     hipLaunchKernelGGL((k_scatter<CT, false>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                       ps.hist, ps.perm, ra);
-  const unsigned nfill = (unsigned)std::min<int64_t>((ps.max_subprob + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_subprob_fill, dim3(std::max(1u, nfill)), dim3(256), 0, s, ps.bin_start, ps.sub_start, nbins,
-                     ps.maxsub, ps.nsub, ps.subprob);
+                       sa, ps.perm, ra);
 }
 
 }  // namespace
@@ -394,6 +385,12 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   if ((e = grow_buffer(ps.bin_start, ps.cap_bins, (int64_t)nbins + 1, ps.retired)) != hipSuccess) return e;
   if ((e = grow_buffer(ps.subprob, ps.cap_sub, max_sub, ps.retired)) != hipSuccess) return e;
   if (!ps.nsub && (e = hipMalloc((void**)&ps.nsub, sizeof(int32_t))) != hipSuccess) return e;
+  if (nparts > ps.cap_sum) {
+    // segment sums start at zero; k_seg_scan leaves them at zero
+    if ((e = grow_buffer(ps.scan_sum, ps.cap_sum, nparts, ps.retired)) != hipSuccess) return e;
+    const unsigned nz = (unsigned)std::min<int64_t>((ps.cap_sum + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_i32, dim3(nz), dim3(256), 0, s, ps.scan_sum, ps.cap_sum);
+  }
   ps.max_subprob = (int)max_sub;
 
   if (M == 0) {
@@ -434,8 +431,8 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
 }
 
 void pointset_free(PointSet& ps) {
-  void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.bin_start, ps.sub_start, ps.subprob, ps.nsub,
-                  ps.recs, ps.sub_off, ps.cmax};
+  void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.scan_sum, ps.bin_start, ps.sub_start, ps.subprob,
+                  ps.nsub, ps.recs, ps.sub_off, ps.cmax};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (void* p : ps.retired) (void)hipFree(p);
@@ -451,6 +448,8 @@ void pointset_free(PointSet& ps) {
   ps.sub_start = nullptr;
   ps.hist = nullptr;
   ps.scan_part = nullptr;
+  ps.scan_sum = nullptr;
+  ps.cap_sum = 0;
   ps.bin_start = nullptr;
   ps.subprob = nullptr;
   ps.nsub = nullptr;
