@@ -52,7 +52,10 @@ struct PointSet {
   // by sub-bin inside every subproblem for the matrix-core kernels (prep.hip)
   bool has_recs = false;
   bool recs_subbin = false;
-  float4* recs = nullptr;        // M records {t_x, t_y, meta, p}
+  float4* recs = nullptr;        // M full records {t_x, t_y, meta, p} (matrix-core kernels)
+  bool has_crec = false;
+  uint2* crec = nullptr;         // M compact records (LDS-tiled kernels)
+  int64_t cap_crec = 0;
   int32_t* sub_off = nullptr;    // max_subprob * 17 sub-bin starts per subproblem
   float* cmax = nullptr;          // n_trans: max |re|, |im| of the strengths (matrix-core spread)
   int64_t cap_recs = 0, cap_sub_off = 0, cap_cmax = 0;
@@ -72,10 +75,11 @@ int lds_budget_bytes();
 int lds_device_max_bytes();
 
 // ---- sort.hip
-// want_recs (2D only): also write the point records; sub_sb: sub-bin side of the
-// matrix-core geometry (16 - ns) to record sub-bins in them, 0 otherwise
+// rec_kind (2D only): also write point records, 1 = full (matrix-core kernels),
+// 2 = compact (LDS-tiled kernels); sub_sb: sub-bin side of the matrix-core
+// geometry (16 - ns) to record sub-bins in full records, 0 otherwise
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
-                         bool want_recs = false, int sub_sb = 0);
+                         int rec_kind = 0, int sub_sb = 0);
 
 // Device buffer of at least `need` elements (contents not kept when it grows).  The
 // outgrown buffer is not freed here: hipFree waits for the whole device, which would

@@ -22,7 +22,7 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
     if (tiled) {
       auto kern = k_gather_tiled<T, CT, D, NS, false>;
       if constexpr (D == 2 && std::is_same<T, float>::value)
-        if (a.recs) kern = k_gather_tiled<T, CT, D, NS, false, true>;
+        if (a.crec) kern = k_gather_tiled<T, CT, D, NS, false, true>;
       if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
                          static_cast<const cplx<T>*>(fw), o);

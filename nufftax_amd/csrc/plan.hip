@@ -344,13 +344,14 @@ int check_plan(nufftx_plan p) {
   return NUFFTX_OK;
 }
 
-// interp: the point set is interpolated to (type 2, grid operators, type-3 targets);
-// only then do the point records pay for their cost in the bin-sort.
+// interp: the point set is interpolated to (type 2, grid operators, type-3 targets).
 int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g, bool interp) {
   std::string err;
   const bool subsort = subsort_applicable(p->tune, g, p->prec);
-  const bool recs = g.dim == 2 && p->prec == kF32 && (interp || subsort);
-  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, recs, subsort ? 16 - g.ns : 0);
+  // records pay for their cost in the sort only where the point set is interpolated;
+  // the matrix-core kernels need the full records with sub-bins
+  const int rec_kind = g.dim == 2 && p->prec == kF32 ? (subsort ? 1 : interp ? 2 : 0) : 0;
+  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, rec_kind, subsort ? 16 - g.ns : 0);
   if (e != hipSuccess) {
     if (!err.empty()) return fail(NUFFTX_ERR_INVALID_ARG, err);
     return fail(NUFFTX_ERR_HIP, std::string("bin-sort: ") + hipGetErrorString(e));

@@ -23,7 +23,7 @@ namespace nufftx {
 template <typename T>
 struct SIArgs {
   const int32_t* perm;
-  const float4* recs;  // 2D FP32 point records in sorted order, or null (sort.hip)
+  const uint2* crec;   // 2D FP32 compact point records in sorted order, or null (sort.hip)
   const float* cmax;   // FP32 spreading: max |re|, |im| of the strengths per transform
   const int4* subprob;
   const int32_t* nsub;
@@ -154,8 +154,15 @@ struct TileAcc<float> {
   }
 };
 
-// REC (2D FP32): points come from the sorted point records (coalesced) instead of
-// perm + a gather and refold of the coordinates.
+// REC (2D FP32): points come from the sorted compact records (coalesced) instead of
+// a gather and refold of the coordinates through perm.  A record word holds the
+// support start in the tile (bits 25..31) and t + ns/2 + 1 in units of 2^-24.
+template <typename T, int NS>
+__device__ __forceinline__ void unpack_compact(unsigned w, int& l, T& t) {
+  l = (int)(w >> 25);
+  t = (T)((double)(w & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1));
+}
+
 template <typename T, typename CT, int D, int NS, bool REC = false>
 __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>* __restrict__ c,
                                                       cplx<T>* __restrict__ fw) {
@@ -195,14 +202,9 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
   int rot[3];
   lane_rotation<NS>(rot);
   for (int j = threadIdx.x; j < sd.z && finite; j += blockDim.x) {
-    float4 rc;
-    int p;
-    if constexpr (REC) {
-      rc = a.recs[sd.y + j];
-      p = __float_as_int(rc.w);
-    } else {
-      p = a.perm[sd.y + j];
-    }
+    uint2 rc;
+    if constexpr (REC) rc = a.crec[sd.y + j];
+    const int p = a.perm[sd.y + j];
     if ((unsigned)p >= (unsigned)a.M) continue;
     const cplx<T> cv = cT[p];
     T w0[NS], w1[NS];
@@ -210,8 +212,7 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
     T tx, ty;
     int l0;
     if constexpr (REC) {
-      l0 = __float_as_int(rc.z) & 4095;
-      tx = (T)rc.x;
+      unpack_compact<T, NS>(rc.x, l0, tx);
 #pragma unroll
       for (int k = 0; k < NS; ++k) w0[k] = es_phi(tx + (T)rot_off<NS>(k, rot[0]), a.es.beta_log2e, a.es.c);
     } else {
@@ -242,8 +243,7 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
     } else {
       int l1;
       if constexpr (REC) {
-        l1 = (__float_as_int(rc.z) >> 12) & 4095;
-        ty = (T)rc.y;
+        unpack_compact<T, NS>(rc.y, l1, ty);
 #pragma unroll
         for (int k = 0; k < NS; ++k) w1[k] = es_phi(ty + (T)rot_off<NS>(k, rot[1]), a.es.beta_log2e, a.es.c);
       } else {
@@ -483,15 +483,15 @@ __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>
     int l[3] = {0, 0, 0};
     int p;
     if constexpr (REC) {
-      const float4 rc = a.recs[sd4.y + j];
-      p = __float_as_int(rc.w);
-      const int meta = __float_as_int(rc.z);
-      l[0] = meta & 4095;
-      l[1] = (meta >> 12) & 4095;
+      const uint2 rc = a.crec[sd4.y + j];
+      p = a.perm[sd4.y + j];
+      T tx, ty;
+      unpack_compact<T, NS>(rc.x, l[0], tx);
+      unpack_compact<T, NS>(rc.y, l[1], ty);
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        w[0][k] = es_phi((T)rc.x + (T)k, a.es.beta_log2e, a.es.c);
-        w[D - 1][k] = es_phi((T)rc.y + (T)k, a.es.beta_log2e, a.es.c);
+        w[0][k] = es_phi(tx + (T)k, a.es.beta_log2e, a.es.c);
+        w[D - 1][k] = es_phi(ty + (T)k, a.es.beta_log2e, a.es.c);
       }
     } else {
       p = a.perm[sd4.y + j];
@@ -561,7 +561,7 @@ SIArgs<T> make_si_args(const GridOpArgs& ga) {
   const Geometry& g = *ga.g;
   SIArgs<T> a;
   a.perm = ps.perm;
-  a.recs = (std::is_same<T, float>::value && g.dim == 2 && ps.has_recs) ? ps.recs : nullptr;
+  a.crec = (std::is_same<T, float>::value && g.dim == 2 && ps.has_crec) ? ps.crec : nullptr;
   a.cmax = nullptr;
   a.subprob = ps.subprob;
   a.nsub = ps.nsub;

@@ -116,15 +116,22 @@ struct ScanArgs {
 // one block: S -> P (exclusive over segments, S cleared); bin totals -> bin_start;
 // subproblems per bin -> sub_start; bin_start[nbins] = M; nsub.
 __global__ void __launch_bounds__(1024) k_seg_scan(ScanArgs sa) {
+  constexpr int kG = 16;  // segment sums loaded together (independent loads in flight)
   for (int b = threadIdx.x; b < sa.nbins; b += blockDim.x) {
     int run = 0;
-#pragma unroll 8
-    for (int g = 0; g < sa.nseg; ++g) {
-      const int64_t i = (int64_t)g * sa.nbins + b;
-      const int t = sa.S[i];
-      sa.S[i] = 0;
-      sa.P[i] = run;
-      run += t;
+    for (int g0 = 0; g0 < sa.nseg; g0 += kG) {
+      int v[kG];
+#pragma unroll
+      for (int u = 0; u < kG; ++u) v[u] = g0 + u < sa.nseg ? sa.S[(int64_t)(g0 + u) * sa.nbins + b] : 0;
+#pragma unroll
+      for (int u = 0; u < kG; ++u) {
+        if (g0 + u < sa.nseg) {
+          const int64_t i = (int64_t)(g0 + u) * sa.nbins + b;
+          sa.S[i] = 0;
+          sa.P[i] = run;
+          run += v[u];
+        }
+      }
     }
     sa.bin_start[b] = run;
   }
@@ -192,15 +199,20 @@ __global__ void __launch_bounds__(1024) k_keys(KeyArgs ka, int64_t M, int CH, in
 
 // Point records of 2D FP32 point sets, written at each point's sorted position so
 // that the spread/interp kernels read them contiguously instead of gathering the
-// coordinates through perm (a random gather costs a 128-byte line per value):
-//   {t_x, t_y, meta, i}, t = i0 - xs (FP32 from the FP64 fold, i0 = ceil(xs - ns/2)),
-//   meta = ox | oy << 12 | s << 24 with (ox, oy) = i0 - (bin origin - ns/2), the support
-//   start in the bin's tile, and s the 4 x 4 sub-bin of the matrix-core geometry
-//   (bins of 4 x 4 sub-bins of sb = 16 - ns cells; 0 when sb == 0).
+// coordinates through perm (a random gather costs a 128-byte line per value).
+// Compact records (the LDS-tiled kernels), 8 bytes: one word per dimension,
+//   word = o << 25 | u,  o = i0 - (bin origin - ns/2) in [0, bs] (support start in
+//   the bin's tile, bs <= 127), u = round((t + ns/2 + 1) 2^24) with t = i0 - xs in
+//   [-ns/2 - 1, -ns/2 + 1) from the FP64 fold (i0 = ceil(xs - ns/2));
+// the point's index is perm at the same slot.  Full records (the matrix-core
+// kernels), 16 bytes: {t_x, t_y, meta, i} with t in FP32, meta = ox | oy << 12 | s << 24
+// and s the 4 x 4 sub-bin of the matrix-core geometry (bins of 4 x 4 sub-bins of
+// sb = 16 - ns cells).
 struct RecArgs {
   const void* coord[2];
   int nf[2], bs[2], nb0, ns, sb;
-  float4* recs;
+  float4* recs;  // full records, or null
+  uint2* crec;   // compact records, or null
 };
 
 __device__ __forceinline__ float4 point_record(const RecArgs& ra, int key, double x, double y, int i) {
@@ -216,9 +228,24 @@ __device__ __forceinline__ float4 point_record(const RecArgs& ra, int key, doubl
                      __int_as_float(ox | (oy << 12) | (sub << 24)), __int_as_float(i));
 }
 
+__device__ __forceinline__ unsigned compact_word(double x, int nf, int bin0, int half) {
+  const Folded f = fold_coord(x, nf);
+  const int i0 = support_start(f, half);
+  const double u = ((double)i0 - f.xs + (double)(half + 1)) * 16777216.0;  // 2^24
+  const unsigned q = (unsigned)min(max(rint(u), 0.0), 33554431.0);      // 2^25 - 1
+  return ((unsigned)(i0 - bin0 + half) << 25) | q;
+}
+
+__device__ __forceinline__ uint2 compact_record(const RecArgs& ra, int key, double x, double y) {
+  const int bx = key % ra.nb0, by = key / ra.nb0;
+  const int half = ra.ns / 2;
+  return make_uint2(compact_word(x, ra.nf[0], bx * ra.bs[0], half), compact_word(y, ra.nf[1], by * ra.bs[1], half));
+}
+
 // k_scatter: W waves per chunk; wave w owns keys [lo + w*SC, lo + (w+1)*SC).
-// REC: also write the 2D point records (coordinates read in index order).
-template <typename CT, bool REC>
+// REC: also write the 2D point records (coordinates read in index order), 1 = full,
+// 2 = compact.
+template <typename CT, int REC>
 __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ keys, int64_t M, int SC, int nbins,
                                                   int key_bits, ScanArgs sa, int32_t* __restrict__ perm, RecArgs ra) {
   extern __shared__ int32_t wcnt[];  // [W][nbins]
@@ -291,10 +318,13 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
       const int pos = b0 + rank;
       if (valid && pos >= 0 && (int64_t)pos < M) {
         perm[pos] = (int32_t)i;
-        if constexpr (REC) {
+        if constexpr (REC != 0) {
           const double x = (double)static_cast<const CT*>(ra.coord[0])[i];
           const double y = (double)static_cast<const CT*>(ra.coord[1])[i];
-          ra.recs[pos] = point_record(ra, key, x, y, (int)i);
+          if constexpr (REC == 1)
+            ra.recs[pos] = point_record(ra, key, x, y, (int)i);
+          else
+            ra.crec[pos] = compact_record(ra, key, x, y);
         }
       }
     }
@@ -334,17 +364,20 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sa);
   const size_t lds = (size_t)W * nbins * sizeof(int32_t);
   if (D == 2 && ra.recs)
-    hipLaunchKernelGGL((k_scatter<CT, true>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                       sa, ps.perm, ra);
+    hipLaunchKernelGGL((k_scatter<CT, 1>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
+                       ps.perm, ra);
+  else if (D == 2 && ra.crec)
+    hipLaunchKernelGGL((k_scatter<CT, 2>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
+                       ps.perm, ra);
   else
-    hipLaunchKernelGGL((k_scatter<CT, false>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits,
-                       sa, ps.perm, ra);
+    hipLaunchKernelGGL((k_scatter<CT, 0>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
+                       ps.perm, ra);
 }
 
 }  // namespace
 
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
-                         bool want_recs, int sub_sb) {
+                         int rec_kind, int sub_sb) {
   hipError_t e;
   const int nbins = g.nbins;
   if ((int64_t)nbins * sizeof(int32_t) > 65536) {
@@ -374,9 +407,12 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   const int64_t nparts = nseg * nbins;
   const int64_t max_sub = nbins + (M + ps.maxsub - 1) / ps.maxsub;
   if ((e = grow_buffer(ps.perm, ps.cap_M, M, ps.retired)) != hipSuccess) return e;
-  const bool recs = want_recs && g.dim == 2 && M > 0;
+  const bool recs = rec_kind == 1 && g.dim == 2 && M > 0;
+  const bool crec = rec_kind == 2 && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
   if (recs && (e = grow_buffer(ps.recs, ps.cap_recs, M, ps.retired)) != hipSuccess) return e;
+  if (crec && (e = grow_buffer(ps.crec, ps.cap_crec, M, ps.retired)) != hipSuccess) return e;
   ps.has_recs = false;
+  ps.has_crec = false;
   ps.recs_subbin = false;
   if ((e = grow_buffer(ps.keys, ps.cap_keys, M, ps.retired)) != hipSuccess) return e;
   if ((e = grow_buffer(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1, ps.retired)) != hipSuccess) return e;
@@ -416,6 +452,7 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   ra.ns = g.ns;
   ra.sb = sub_sb;
   ra.recs = recs ? ps.recs : nullptr;
+  ra.crec = crec ? ps.crec : nullptr;
   if (ps.coord_f64) {
     if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, ra, s);
     if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, ra, s);
@@ -427,12 +464,13 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   }
   e = hipGetLastError();
   if (e == hipSuccess && recs) ps.has_recs = true;
+  if (e == hipSuccess && crec) ps.has_crec = true;
   return e;
 }
 
 void pointset_free(PointSet& ps) {
   void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.scan_sum, ps.bin_start, ps.sub_start, ps.subprob,
-                  ps.nsub, ps.recs, ps.sub_off, ps.cmax};
+                  ps.nsub, ps.recs, ps.crec, ps.sub_off, ps.cmax};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (void* p : ps.retired) (void)hipFree(p);
@@ -440,6 +478,9 @@ void pointset_free(PointSet& ps) {
   ps.perm = nullptr;
   ps.keys = nullptr;
   ps.recs = nullptr;
+  ps.crec = nullptr;
+  ps.has_crec = false;
+  ps.cap_crec = 0;
   ps.sub_off = nullptr;
   ps.cmax = nullptr;
   ps.has_recs = false;

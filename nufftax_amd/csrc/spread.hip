@@ -40,7 +40,7 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
     if (tiled) {
       auto kern = k_spread_tiled<T, CT, D, NS>;
       if constexpr (D == 2 && std::is_same<T, float>::value)
-        if (a.recs) kern = k_spread_tiled<T, CT, D, NS, true>;
+        if (a.crec) kern = k_spread_tiled<T, CT, D, NS, true>;
       if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
                          static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
