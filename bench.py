@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # the same counters read for a 256 MiB copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
 # the summary with the commands is profiles/nufft2d_radial_4M_hbm_traffic.md.
 TRAFFIC_BYTES = {
-    "k_spread_tiled<float,float,2,8>": 494261.3 * 1024 * 2.0 + 78768.4 * 1024,          # 1.09 GB
-    "k_gather_tiled<float,float,2,8,false,true>": 59159.2 * 1024 * 2.0 + 123500.8 * 1024,  # 0.25 GB
+    "k_spread_tiled<float,float,2,8>": 494712.8 * 1024 * 2.0 + 78768.4 * 1024,          # 1.09 GB
+    "k_gather_tiled<float,float,2,8,false,true>": 51741.6 * 1024 * 2.0 + 123568.5 * 1024,  # 0.23 GB
 }
 
 
