@@ -38,6 +38,18 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
   return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;
     if (tiled) {
+      if constexpr (D == 2 && std::is_same<T, float>::value) {
+        // 2D FP32: the bank-spread kernel when its point stage fits (maxsub <= 65535 points,
+        // supports within 127 cells of the tile origin)
+        const size_t lb = banked2d_lds(a.tile_cells, ps.maxsub);
+        if (ps.maxsub <= 65535 && lb <= (size_t)lds_device_max_bytes() && a.bs[0] <= 127 && a.bs[1] <= 127) {
+          auto kb = a.crec ? k_spread_banked2d<CT, NS, true, kBankClasses> : k_spread_banked2d<CT, NS, false, kBankClasses>;
+          if (lb > 65536) (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
+          hipLaunchKernelGGL(kb, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lb, s, a, ps.maxsub,
+                             static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
+          return hipGetLastError();
+        }
+      }
       auto kern = k_spread_tiled<T, CT, D, NS>;
       if constexpr (D == 2 && std::is_same<T, float>::value)
         if (a.crec) kern = k_spread_tiled<T, CT, D, NS, true>;
