@@ -331,6 +331,51 @@ __device__ __forceinline__ unsigned pack_compact(double x, int nf, int org) {
   return ((unsigned)(i0 - org) << 25) | q;
 }
 
+// Phases 1 and 2 of the bank-spread kernels: stage the n points of subproblem sd in
+// LDS (compact record words sw, indices sp) and write their class order into `order`.
+// ccnt[C] must be zero and visible to the block on entry; ends with a barrier.
+template <typename CT, int NS, bool REC, int C>
+__device__ __forceinline__ void bank_order(const SIArgs<float>& a, const int4& sd, const int (&org)[3], uint2* sw,
+                                           int* sp, unsigned short* order, int* ccnt, int* cfill) {
+  const int n = sd.z, t0 = a.tile[0];
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const int p = a.perm[sd.y + j];
+    uint2 w = make_uint2(0u, 0u);
+    if constexpr (REC) {
+      w = a.crec[sd.y + j];
+    } else if ((unsigned)p < (unsigned)a.M) {
+      w.x = pack_compact<NS>((double)static_cast<const CT*>(a.coord[0])[p], a.nf[0], org[0]);
+      w.y = pack_compact<NS>((double)static_cast<const CT*>(a.coord[1])[p], a.nf[1], org[1]);
+    }
+    sw[j] = w;
+    sp[j] = p;
+    atomicAdd(&ccnt[((int)(w.y >> 25) * t0 + (int)(w.x >> 25)) & (C - 1)], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {  // exclusive scan of the class counts (one wave)
+    const int lane = threadIdx.x;
+    int carry = 0;
+    for (int base = 0; base < C; base += 64) {
+      const int v = base + lane < C ? ccnt[base + lane] : 0;
+      int incl = v;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      if (base + lane < C) cfill[base + lane] = carry + incl - v;
+      carry += __shfl(incl, 63, 64);
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const uint2 w = sw[j];
+    const int slot = atomicAdd(&cfill[((int)(w.y >> 25) * t0 + (int)(w.x >> 25)) & (C - 1)], 1);
+    order[slot] = (unsigned short)j;
+  }
+  __syncthreads();
+}
+
 template <typename CT, int NS, bool REC, int C>
 __global__ void __launch_bounds__(256) k_spread_banked2d(SIArgs<float> a, int maxsub, const cplx<float>* __restrict__ c,
                                                          cplx<float>* __restrict__ fw) {
@@ -365,45 +410,8 @@ __global__ void __launch_bounds__(256) k_spread_banked2d(SIArgs<float> a, int ma
     inv_scale = ldexp(1.0, e - 50);
   }
   __syncthreads();
+  bank_order<CT, NS, REC, C>(a, sd, org, sw, sp, order, ccnt, cfill);
   const int t0 = a.tile[0];
-  // ---- phase 1: stage the points, count their classes
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const int p = a.perm[sd.y + j];
-    uint2 w = make_uint2(0u, 0u);
-    if constexpr (REC) {
-      w = a.crec[sd.y + j];
-    } else if ((unsigned)p < (unsigned)a.M) {
-      w.x = pack_compact<NS>((double)static_cast<const CT*>(a.coord[0])[p], a.nf[0], org[0]);
-      w.y = pack_compact<NS>((double)static_cast<const CT*>(a.coord[1])[p], a.nf[1], org[1]);
-    }
-    sw[j] = w;
-    sp[j] = p;
-    atomicAdd(&ccnt[((int)(w.y >> 25) * t0 + (int)(w.x >> 25)) & (C - 1)], 1);
-  }
-  __syncthreads();
-  if (threadIdx.x < 64) {  // exclusive scan of the class counts (one wave)
-    const int lane = threadIdx.x;
-    int carry = 0;
-    for (int base = 0; base < C; base += 64) {
-      const int v = base + lane < C ? ccnt[base + lane] : 0;
-      int incl = v;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int t = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += t;
-      }
-      if (base + lane < C) cfill[base + lane] = carry + incl - v;
-      carry += __shfl(incl, 63, 64);
-    }
-  }
-  __syncthreads();
-  // ---- phase 2: order the staged points by class
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const uint2 w = sw[j];
-    const int slot = atomicAdd(&cfill[((int)(w.y >> 25) * t0 + (int)(w.x >> 25)) & (C - 1)], 1);
-    order[slot] = (unsigned short)j;
-  }
-  __syncthreads();
   // ---- phase 3: accumulate, lanes dealt across the classes
   const cplx<float>* cT = c + (int64_t)tr * a.M;
   const int S = (n + C - 1) / C;
