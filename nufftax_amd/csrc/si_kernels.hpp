@@ -301,24 +301,25 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
   }
 }
 
-// ------------------------------------------------------------------ spread (2D FP32, bank-spread)
-// Same sums as k_spread_tiled<float, CT, 2, NS> (the 64-bit fixed-point tile is exact, so
+// ------------------------------------------------------------------ spread (FP32, bank-spread)
+// Same sums as k_spread_tiled<float, CT, D, NS> (the 64-bit fixed-point tile is exact, so
 // the order in which a subproblem's points are added does not matter), with the points
 // dealt to the lanes so that a wave's 64 atomics per step land on distinct LDS banks.
-// Support cell (ky, kx) of a point is at (l1 + ky) t0 + l0 + kx, so at every step the
-// lanes' addresses differ by their bases l1 t0 + l0: bases covering the classes
-// (base mod C) evenly give conflict-free atomics, where random bases serialise on shared
-// banks.  Phase 1 stages every point in LDS (its two compact record words, see
-// unpack_compact, and its index) and counts the classes, phase 2 orders the staged points
-// by class, phase 3 walks the class-ordered list transposed (lane q takes entry
-// (q mod C) S + q / C, S = ceil(n / C)), so consecutive lanes take consecutive classes.
-// 14 bytes of LDS per point: with maxsub = 1024 and a 40 x 40 tile, 4 blocks fit a CU.
+// Support cell (kz, ky, kx) of a point is at base + kz t0 t1 + ky t0 + kx with
+// base = l2 t0 t1 + l1 t0 + l0, so at every step the lanes' addresses differ by their
+// bases: bases covering the classes (base mod C) evenly give conflict-free atomics, where
+// random bases serialise on shared banks.  Phase 1 stages every point in LDS (one compact
+// record word per dimension, see unpack_compact, and its index) and counts the classes,
+// phase 2 orders the staged points by class, phase 3 walks the class-ordered list
+// transposed (lane q takes entry (q mod C) S + q / C, S = ceil(n / C)), so consecutive
+// lanes take consecutive classes.
 // C = 16 classes measured best on gfx950 (32 four-byte banks: 16 pairs for the 64-bit
-// atomics; 32 and 64 classes were 2% and 9% slower on the 2D radial workload).
+// atomics; 32 and 64 classes were 2% and 9% slower on a 2D radial workload).
 constexpr int kBankClasses = 16;
 
-inline size_t banked2d_lds(int64_t tile_cells, int maxsub) {
-  return (size_t)tile_cells * 16 + (size_t)maxsub * (sizeof(uint2) + sizeof(int) + sizeof(unsigned short));
+template <int D>
+inline size_t banked_lds(int64_t tile_cells, int maxsub) {
+  return (size_t)tile_cells * 16 + (size_t)maxsub * (D * sizeof(unsigned) + sizeof(int) + sizeof(unsigned short));
 }
 
 // compact record word of one coordinate (the layout of unpack_compact), tile-relative
@@ -331,25 +332,39 @@ __device__ __forceinline__ unsigned pack_compact(double x, int nf, int org) {
   return ((unsigned)(i0 - org) << 25) | q;
 }
 
-// Phases 1 and 2 of the bank-spread kernels: stage the n points of subproblem sd in
-// LDS (compact record words sw, indices sp) and write their class order into `order`.
-// ccnt[C] must be zero and visible to the block on entry; ends with a barrier.
-template <typename CT, int NS, bool REC, int C>
-__device__ __forceinline__ void bank_order(const SIArgs<float>& a, const int4& sd, const int (&org)[3], uint2* sw,
-                                           int* sp, unsigned short* order, int* ccnt, int* cfill) {
-  const int n = sd.z, t0 = a.tile[0];
+template <int D>
+__device__ __forceinline__ int compact_base(const unsigned (&w)[D], int t0, int t01) {
+  int b = (int)(w[0] >> 25);
+  if constexpr (D >= 2) b += (int)(w[1] >> 25) * t0;
+  if constexpr (D >= 3) b += (int)(w[2] >> 25) * t01;
+  return b;
+}
+
+// Phases 1 and 2 of the bank-spread kernel: stage the n points of subproblem sd in LDS
+// (word d of point j at sw[d maxsub + j], index at sp[j]) and write their class order into
+// `order`.  ccnt[C] must be zero and visible to the block on entry; ends with a barrier.
+template <typename CT, int D, int NS, bool REC, int C>
+__device__ __forceinline__ void bank_order(const SIArgs<float>& a, const int4& sd, const int (&org)[3], int maxsub,
+                                           unsigned* sw, int* sp, unsigned short* order, int* ccnt, int* cfill) {
+  const int n = sd.z, t0 = a.tile[0], t01 = a.tile[0] * a.tile[1];
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     const int p = a.perm[sd.y + j];
-    uint2 w = make_uint2(0u, 0u);
+    unsigned w[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) w[d] = 0u;
     if constexpr (REC) {
-      w = a.crec[sd.y + j];
+      static_assert(D == 2, "compact records exist for 2D point sets");
+      const uint2 r = a.crec[sd.y + j];
+      w[0] = r.x;
+      w[1] = r.y;
     } else if ((unsigned)p < (unsigned)a.M) {
-      w.x = pack_compact<NS>((double)static_cast<const CT*>(a.coord[0])[p], a.nf[0], org[0]);
-      w.y = pack_compact<NS>((double)static_cast<const CT*>(a.coord[1])[p], a.nf[1], org[1]);
+#pragma unroll
+      for (int d = 0; d < D; ++d) w[d] = pack_compact<NS>((double)static_cast<const CT*>(a.coord[d])[p], a.nf[d], org[d]);
     }
-    sw[j] = w;
+#pragma unroll
+    for (int d = 0; d < D; ++d) sw[d * maxsub + j] = w[d];
     sp[j] = p;
-    atomicAdd(&ccnt[((int)(w.y >> 25) * t0 + (int)(w.x >> 25)) & (C - 1)], 1);
+    atomicAdd(&ccnt[compact_base<D>(w, t0, t01) & (C - 1)], 1);
   }
   __syncthreads();
   if (threadIdx.x < 64) {  // exclusive scan of the class counts (one wave)
@@ -369,22 +384,24 @@ __device__ __forceinline__ void bank_order(const SIArgs<float>& a, const int4& s
   }
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const uint2 w = sw[j];
-    const int slot = atomicAdd(&cfill[((int)(w.y >> 25) * t0 + (int)(w.x >> 25)) & (C - 1)], 1);
+    unsigned w[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) w[d] = sw[d * maxsub + j];
+    const int slot = atomicAdd(&cfill[compact_base<D>(w, t0, t01) & (C - 1)], 1);
     order[slot] = (unsigned short)j;
   }
   __syncthreads();
 }
 
-template <typename CT, int NS, bool REC, int C>
-__global__ void __launch_bounds__(256) k_spread_banked2d(SIArgs<float> a, int maxsub, const cplx<float>* __restrict__ c,
-                                                         cplx<float>* __restrict__ fw) {
+template <typename CT, int D, int NS, bool REC, int C>
+__global__ void __launch_bounds__(1024) k_spread_banked(SIArgs<float> a, int maxsub, const cplx<float>* __restrict__ c,
+                                                        cplx<float>* __restrict__ fw) {
   using Word = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Word* sre = reinterpret_cast<Word*>(smem_raw);
   Word* sim = sre + a.tile_cells;
-  uint2* sw = reinterpret_cast<uint2*>(sim + a.tile_cells);
-  int* sp = reinterpret_cast<int*>(sw + maxsub);
+  unsigned* sw = reinterpret_cast<unsigned*>(sim + a.tile_cells);
+  int* sp = reinterpret_cast<int*>(sw + (size_t)D * maxsub);
   unsigned short* order = reinterpret_cast<unsigned short*>(sp + maxsub);
   __shared__ int ccnt[C], cfill[C];
   const int sub = blockIdx.x;
@@ -393,7 +410,7 @@ __global__ void __launch_bounds__(256) k_spread_banked2d(SIArgs<float> a, int ma
   const int4 sd = a.subprob[sub];
   const int n = sd.z;
   int org[3];
-  tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
+  tile_origin<D>(sd, a.bs, a.nb, NS / 2, org);
   for (int64_t i = threadIdx.x; i < a.tile_cells; i += blockDim.x) {
     sre[i] = 0;
     sim[i] = 0;
@@ -410,8 +427,8 @@ __global__ void __launch_bounds__(256) k_spread_banked2d(SIArgs<float> a, int ma
     inv_scale = ldexp(1.0, e - 50);
   }
   __syncthreads();
-  bank_order<CT, NS, REC, C>(a, sd, org, sw, sp, order, ccnt, cfill);
-  const int t0 = a.tile[0];
+  bank_order<CT, D, NS, REC, C>(a, sd, org, maxsub, sw, sp, order, ccnt, cfill);
+  const int t0 = a.tile[0], t01 = a.tile[0] * a.tile[1];
   // ---- phase 3: accumulate, lanes dealt across the classes
   const cplx<float>* cT = c + (int64_t)tr * a.M;
   const int S = (n + C - 1) / C;
@@ -423,28 +440,38 @@ __global__ void __launch_bounds__(256) k_spread_banked2d(SIArgs<float> a, int ma
     const int p = sp[j];
     if ((unsigned)p >= (unsigned)a.M) continue;
     const cplx<float> cv = cT[p];
-    const uint2 w = sw[j];
-    int l0, l1;
-    float tx, ty;
-    unpack_compact<float, NS>(w.x, l0, tx);
-    unpack_compact<float, NS>(w.y, l1, ty);
-    const int base = l1 * t0 + l0;
+    int l[3] = {0, 0, 0};
+    float t[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int d = 0; d < D; ++d) unpack_compact<float, NS>(sw[d * maxsub + j], l[d], t[d]);
+    const int base = l[0] + l[1] * t0 + l[2] * t01;
     double wd[NS];
     float w1[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      wd[k] = (double)es_phi(tx + (float)k, a.es.beta_log2e, a.es.c);
-      w1[k] = es_phi(ty + (float)k, a.es.beta_log2e, a.es.c);
+      wd[k] = (double)es_phi(t[0] + (float)k, a.es.beta_log2e, a.es.c);
+      w1[k] = es_phi(t[1] + (float)k, a.es.beta_log2e, a.es.c);
     }
     const double cre = (double)cv.re * scale, cim = (double)cv.im * scale;
+    const int kzn = D == 3 ? NS : 1;
+#pragma unroll 1
+    for (int kz = 0; kz < kzn; ++kz) {
+      double zr = cre, zi = cim;
+      if constexpr (D == 3) {
+        const double wz = (double)es_phi(t[2] + (float)kz, a.es.beta_log2e, a.es.c);
+        zr *= wz;
+        zi *= wz;
+      }
+      const int plane = base + kz * t01;
 #pragma unroll
-    for (int ky = 0; ky < NS; ++ky) {
-      const double vr = cre * (double)w1[ky], vi = cim * (double)w1[ky];
-      const int row = base + ky * t0;
+      for (int ky = 0; ky < NS; ++ky) {
+        const double vr = zr * (double)w1[ky], vi = zi * (double)w1[ky];
+        const int row = plane + ky * t0;
 #pragma unroll
-      for (int kx = 0; kx < NS; ++kx) {
-        atomicAdd(sre + row + kx, TileAcc<float>::quant(vr, wd[kx]));
-        atomicAdd(sim + row + kx, TileAcc<float>::quant(vi, wd[kx]));
+        for (int kx = 0; kx < NS; ++kx) {
+          atomicAdd(sre + row + kx, TileAcc<float>::quant(vr, wd[kx]));
+          atomicAdd(sim + row + kx, TileAcc<float>::quant(vi, wd[kx]));
+        }
       }
     }
   }
@@ -455,7 +482,7 @@ __global__ void __launch_bounds__(256) k_spread_banked2d(SIArgs<float> a, int ma
     if (finite && ir == 0 && ii == 0) continue;
     const float vr = finite ? (float)((double)ir * inv_scale) : NAN;
     const float vi = finite ? (float)((double)ii * inv_scale) : NAN;
-    const int64_t gi = tile_cell_to_global<2>(cell, a.tile, org, a.nf);
+    const int64_t gi = tile_cell_to_global<D>(cell, a.tile, org, a.nf);
     atomic_add_global(&fT[gi].re, vr);
     atomic_add_global(&fT[gi].im, vi);
   }

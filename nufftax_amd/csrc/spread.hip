@@ -38,14 +38,20 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
   return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;
     if (tiled) {
-      if constexpr (D == 2 && std::is_same<T, float>::value) {
-        // 2D FP32: the bank-spread kernel when its point stage fits (maxsub <= 65535 points,
-        // supports within 127 cells of the tile origin)
-        const size_t lb = banked2d_lds(a.tile_cells, ps.maxsub);
-        if (ps.maxsub <= 65535 && lb <= (size_t)lds_device_max_bytes() && a.bs[0] <= 127 && a.bs[1] <= 127) {
-          auto kb = a.crec ? k_spread_banked2d<CT, NS, true, kBankClasses> : k_spread_banked2d<CT, NS, false, kBankClasses>;
+      if constexpr (D >= 2 && std::is_same<T, float>::value) {
+        // 2D/3D FP32: the bank-spread kernel when its point stage fits (maxsub <= 65535 points,
+        // supports within 127 cells of the tile origin).  Tiles that leave room for one block
+        // per CU get 1024 threads, the others 256 (several blocks per CU).
+        const size_t lb = banked_lds<D>(a.tile_cells, ps.maxsub);
+        bool fits = ps.maxsub <= 65535 && lb <= (size_t)lds_device_max_bytes();
+        for (int d = 0; d < D; ++d) fits = fits && a.bs[d] <= 127;
+        if (fits) {
+          auto kb = k_spread_banked<CT, D, NS, false, kBankClasses>;
+          if constexpr (D == 2)
+            if (a.crec) kb = k_spread_banked<CT, D, NS, true, kBankClasses>;
           if (lb > 65536) (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
-          hipLaunchKernelGGL(kb, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lb, s, a, ps.maxsub,
+          const unsigned nt = 2 * lb > (size_t)lds_device_max_bytes() ? 1024 : 256;
+          hipLaunchKernelGGL(kb, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(nt), lb, s, a, ps.maxsub,
                              static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
           return hipGetLastError();
         }
