@@ -43,6 +43,7 @@ def _parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D secondary figures")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000, help="points in the CPU-baseline sample")
     ap.add_argument("--cpu-runs", type=int, default=7, help="timed CPU-baseline runs (median reported)")
     ap.add_argument("--profile-only", action="store_true", help="run a few steps, print nothing but a status line")
@@ -99,6 +100,50 @@ def cpu_baseline(x, y, c, f, nm, eps, sample, runs):
         "sample": f"nufft2d1 + nufft2d2 on the first {sample} radial points (512x512 modes, complex64), "
                   f"median of {len(times)} runs, {t:.2f} s each",
     }
+
+
+def bench_3d(args, torch, dev):
+    """3D type 1 + type 2 on 8M points in 16 Gaussian clusters, 128^3 modes, complex64, eps
+    1e-6 and 1e-4: ms per transform (points sorted in every call) and NU pts/s."""
+    from nufftax_amd import workloads
+    from nufftax_amd._lib import lib, ptr, check
+    from nufftax_amd._plan import F32, transform_plan
+
+    coords, c = workloads.gaussian_mixture_3d(seed=0)
+    M = coords[0].size
+    xs = [torch.from_numpy(x.astype(np.float32)).to(dev) for x in coords]
+    cd = torch.from_numpy(c.astype(np.complex64)).to(dev)
+    nm = (128, 128, 128)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    fd = torch.randn(nm, dtype=torch.complex64, device=dev, generator=g)
+    fout = torch.empty(nm, dtype=torch.complex64, device=dev)
+    cout = torch.empty(M, dtype=torch.complex64, device=dev)
+    L = lib()
+    out = {"workload": "3D Gaussian clusters type-1 + type-2", "M": M, "n_modes": list(nm)}
+    for eps in (1e-6, 1e-4):
+        p1 = transform_plan(1, 3, nm, 1, eps, 1, 2.0, F32, dev)
+        p2 = transform_plan(2, 3, nm, 1, eps, -1, 2.0, F32, dev)
+
+        def run(p, a, b):
+            p._sync_stream()
+            check(L.nufftx_setpts(p.h, M, ptr(xs[0]), ptr(xs[1]), ptr(xs[2]), 0, None, None, None), "setpts")
+            check(L.nufftx_execute(p.h, ptr(a), ptr(b)), "execute")
+
+        def timeit(fn, n=10):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(n):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / n
+
+        t1 = timeit(lambda: run(p1, cd, fout))
+        t2 = timeit(lambda: run(p2, cout, fd))
+        out[f"eps_{eps:g}"] = {"nufft3d1_ms": t1 * 1e3, "nufft3d2_ms": t2 * 1e3, "value": 2 * M / (t1 + t2),
+                               "unit": "NU pts/s"}
+    return out
 
 
 def main():
@@ -253,6 +298,17 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": kb,
                      "kernel_ms": kt * 1e3},
     }
+    # whole-transform roofline: minimal HBM bytes of each transform (sort, spread/interp, FFT,
+    # deconvolution) over its measured time
+    nmc = nm[0] * nm[1]
+    b1 = M * (2 * 2 * 4 + 8) + M * 8 + 3 * 8 * nfc + 2 * 8 * nmc
+    b2 = M * (2 * 2 * 4 + 8) + M * 8 + 4 * 8 * nfc + 8 * nmc
+    res["transform_roofline"] = {
+        "nufft2d1": {"bytes": b1, "ms": t_t1 * 1e3, "frac": b1 / t_t1 / (HBM_PEAK_GBS * 1e9)},
+        "nufft2d2": {"bytes": b2, "ms": t_t2 * 1e3, "frac": b2 / t_t2 / (HBM_PEAK_GBS * 1e9)},
+    }
+    if not args.no_3d:
+        res["config_3d"] = bench_3d(args, torch, dev)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(x32, y32, c64, f64, nm, eps, args.cpu_sample, args.cpu_runs)
     print(json.dumps(res), flush=True)

@@ -141,6 +141,21 @@ struct DeconvArgs {
 hipError_t launch_deconv_shuffle(const DeconvArgs& a, const void* fw_hat, void* f, hipStream_t s);
 hipError_t launch_deconv_pad(const DeconvArgs& a, const void* f, void* fw_hat, hipStream_t s);
 
+// ---- fft2d.hip: fused FFT + deconvolution of 2D FP32 plans on power-of-two grids
+struct Fft2Plan {
+  void* tw[2] = {nullptr, nullptr};  // float2 twiddle tables, nf_d / 2 entries
+  float2* mid = nullptr;             // n_trans * nm0 * nf1 intermediate (x-modes, transposed)
+  int64_t cap_mid = 0;
+  std::vector<void*> retired;
+};
+bool fft2_applicable(const DeconvArgs& a);
+hipError_t fft2_plan_init(Fft2Plan& fp, const int* nf);
+void fft2_plan_free(Fft2Plan& fp);
+// grid (FFT of the spread grid) -> deconvolved modes, and modes -> padded, deconvolved,
+// transformed grid; isign selects the exponent sign (both unnormalised)
+hipError_t fft2_shuffle(Fft2Plan& fp, const DeconvArgs& a, const void* grid, void* f, int isign, hipStream_t s);
+hipError_t fft2_pad(Fft2Plan& fp, const DeconvArgs& a, const void* f, void* grid, int isign, hipStream_t s);
+
 // ---- type 3 (elementwise.hip)
 struct T3State {  // lives in device memory, filled by launch_t3_setup
   double C[3], D[3], gamma[3], h[3];

@@ -308,6 +308,8 @@ struct nufftx_plan_s {
   void* fac[3] = {nullptr, nullptr, nullptr};
   void* fac_in[3] = {nullptr, nullptr, nullptr};
   FftPlan fft, fft_in;
+  Fft2Plan fft2;          // fused FFT + deconvolution (2D FP32, power-of-two grids)
+  bool use_fft2 = false;
   QuadRule quad;
   // type 3
   T3State* t3 = nullptr;
@@ -329,6 +331,7 @@ void plan_free(nufftx_plan p) {
   pointset_free(p->ps);
   pointset_free(p->pt);
   fft_free(p->fft);
+  fft2_plan_free(p->fft2);
   fft_free(p->fft_in);
   void* bufs[] = {p->grid, p->grid_in, p->fac[0], p->fac[1], p->fac[2], p->fac_in[0], p->fac_in[1], p->fac_in[2],
                   p->t3, p->red, p->xr[0], p->xr[1], p->xr[2], p->sr[0], p->sr[1], p->sr[2], p->pre, p->post,
@@ -394,18 +397,36 @@ DeconvArgs dargs(nufftx_plan p, bool inner) {
     if (e_ != hipSuccess) return fail(NUFFTX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-int exec_type1(nufftx_plan p, const void* c, void* f, int isign) {
-  const int64_t gbytes = p->g.grid_cells() * p->n_trans * (int64_t)p->csize();
-  LAUNCH(launch_zero(p->grid, gbytes, p->stream));
-  LAUNCH(launch_spread(gop(p, p->ps, p->g), c, p->grid, p->stream));
+// fine grid -> FFT -> deconvolved modes f (type 1 output stage)
+int grid_to_modes(nufftx_plan p, void* f, int isign) {
+  if (p->use_fft2) {
+    LAUNCH(fft2_shuffle(p->fft2, dargs(p, false), p->grid, f, isign, p->stream));
+    return NUFFTX_OK;
+  }
   if (int r = fft_exec(p->fft, p->grid, isign, p->stream)) return r;
   LAUNCH(launch_deconv_shuffle(dargs(p, false), p->grid, f, p->stream));
   return NUFFTX_OK;
 }
 
-int exec_type2(nufftx_plan p, void* c, const void* f, int isign) {
+// modes f -> deconvolved, padded -> FFT -> fine grid (type 2 input stage)
+int modes_to_grid(nufftx_plan p, const void* f, int isign) {
+  if (p->use_fft2) {
+    LAUNCH(fft2_pad(p->fft2, dargs(p, false), f, p->grid, isign, p->stream));
+    return NUFFTX_OK;
+  }
   LAUNCH(launch_deconv_pad(dargs(p, false), f, p->grid, p->stream));
-  if (int r = fft_exec(p->fft, p->grid, isign, p->stream)) return r;
+  return fft_exec(p->fft, p->grid, isign, p->stream);
+}
+
+int exec_type1(nufftx_plan p, const void* c, void* f, int isign) {
+  const int64_t gbytes = p->g.grid_cells() * p->n_trans * (int64_t)p->csize();
+  LAUNCH(launch_zero(p->grid, gbytes, p->stream));
+  LAUNCH(launch_spread(gop(p, p->ps, p->g), c, p->grid, p->stream));
+  return grid_to_modes(p, f, isign);
+}
+
+int exec_type2(nufftx_plan p, void* c, const void* f, int isign) {
+  if (int r = modes_to_grid(p, f, isign)) return r;
   LAUNCH(launch_interp(gop(p, p->ps, p->g), p->grid, c, p->stream));
   return NUFFTX_OK;
 }
@@ -510,6 +531,16 @@ int nufftx_makeplan(int type, int dim, const int64_t* n_modes, int isign, int64_
       if ((r = upload_factors(p->fac[d], deconv_factors(ph, p->n_modes[d], modeord), p->prec))) goto fail_out;
     }
     if ((r = fft_make(p->fft, dim, p->g.nf, n_trans, p->prec))) goto fail_out;
+    // NUFFTX_FUSED_FFT selects the fused radix-2 FFT + deconvolution passes; measured
+    // slower than rocFFT + the deconvolution kernel on 1024^2 (42 against 32 us per
+    // transform), so rocFFT stays the default
+    if (fft2_applicable(dargs(p, false)) && std::getenv("NUFFTX_FUSED_FFT")) {
+      if (fft2_plan_init(p->fft2, p->g.nf) != hipSuccess) {
+        r = fail(NUFFTX_ERR_ALLOC, "fused FFT twiddle tables");
+        goto fail_out;
+      }
+      p->use_fft2 = true;
+    }
   } else {
     char* gi = nullptr;
     const size_t gib = (size_t)p->gi.grid_cells() * p->n_trans * p->csize();
@@ -655,8 +686,7 @@ int nufftx_vjp(nufftx_plan p, const void* src, const void* g, void* grad_src, vo
   const int s = -p->isign;
   if (p->type == 1) {
     // G = FFT_{-isign}(pad(g / phihat)) -- adjoint of deconvolve-shuffle and FFT
-    LAUNCH(launch_deconv_pad(dargs(p, false), g, p->grid, p->stream));
-    if (int r = fft_exec(p->fft, p->grid, s, p->stream)) return r;
+    if (int r = modes_to_grid(p, g, s)) return r;
     if (want_coord) {
       LAUNCH(launch_interp_grad(gop(p, p->ps, p->g), p->grid, src, grad_src, gc, p->stream));
     } else if (grad_src) {
@@ -666,8 +696,7 @@ int nufftx_vjp(nufftx_plan p, const void* src, const void* g, void* grad_src, vo
   }
   // type 2
   if (want_coord) {
-    LAUNCH(launch_deconv_pad(dargs(p, false), src, p->grid, p->stream));
-    if (int r = fft_exec(p->fft, p->grid, p->isign, p->stream)) return r;
+    if (int r = modes_to_grid(p, src, p->isign)) return r;
     LAUNCH(launch_interp_grad(gop(p, p->ps, p->g), p->grid, g, nullptr, gc, p->stream));
   }
   if (grad_src) return exec_type1(p, g, grad_src, s);

@@ -4,19 +4,22 @@
 // Contract: perm == argsort(key, stable) and bin_start == [0, cumsum(bincount(key))],
 // where key = sum_d (floor(xs_d) div bs_d) * prod_{e<d} nb_e and xs is the FP64 fold.
 //
-// Counting sort, all integer work, in three launches:
+// Counting sort, all integer work, in three launches (five above 1024 bins, see B):
 //   A  k_keys: one block per chunk of points folds each point once, stores its
 //      key and counts the keys in LDS (per-lane integer atomics); the block writes
 //      its row of the histogram H[chunk][bin] (coalesced) and adds the row into
 //      the sums of its segment of 16 rows, S[seg][bin] (global integer atomics).
 //   B  k_seg_scan (one block): S -> exclusive prefixes over the segments (P) and
 //      bin totals, the totals -> bin_start and, per bin, the first subproblem
-//      (sub_start); S is cleared for the next sort.
+//      (sub_start); S is cleared for the next sort.  Above 1024 bins the prefixes run
+//      one thread per bin (k_seg_prefix), the bin scan in one block (k_bin_scan), and
+//      k_row_offsets turns every histogram row into first output slots.
 //   C  k_scatter: each block first fills its share of the subproblem descriptors
 //      {bin, first sorted slot, count} (bins split into <= maxsub points) used by
 //      the tiled spread/interp kernels; then the W waves of a chunk count their
 //      share of the chunk's keys, turn the counts into per-wave bases (bin start +
-//      segment prefix + rows of the segment before the chunk), and walk their keys
+//      segment prefix + rows of the segment before the chunk, or the row offsets of
+//      k_row_offsets), and walk their keys
 //      in index order, ranking equal keys inside each 64-key step (one comparison
 //      when the step's keys agree, else one ballot per key bit), and write
 //      perm[slot] = index.
@@ -111,31 +114,33 @@ struct ScanArgs {
   int32_t* nsub;
   int4* subprob;
   int NB, nseg, nbins, M, maxsub, max_subprob;
+  int rows_are_offsets;  // H already holds first output slots (k_row_offsets ran)
 };
 
-// one block: S -> P (exclusive over segments, S cleared); bin totals -> bin_start;
-// subproblems per bin -> sub_start; bin_start[nbins] = M; nsub.
-__global__ void __launch_bounds__(1024) k_seg_scan(ScanArgs sa) {
+// S -> P for bin b (exclusive over the segments, S cleared); the bin total into bin_start[b]
+__device__ __forceinline__ void seg_prefix_bin(const ScanArgs& sa, int b) {
   constexpr int kG = 16;  // segment sums loaded together (independent loads in flight)
-  for (int b = threadIdx.x; b < sa.nbins; b += blockDim.x) {
-    int run = 0;
-    for (int g0 = 0; g0 < sa.nseg; g0 += kG) {
-      int v[kG];
+  int run = 0;
+  for (int g0 = 0; g0 < sa.nseg; g0 += kG) {
+    int v[kG];
 #pragma unroll
-      for (int u = 0; u < kG; ++u) v[u] = g0 + u < sa.nseg ? sa.S[(int64_t)(g0 + u) * sa.nbins + b] : 0;
+    for (int u = 0; u < kG; ++u) v[u] = g0 + u < sa.nseg ? sa.S[(int64_t)(g0 + u) * sa.nbins + b] : 0;
 #pragma unroll
-      for (int u = 0; u < kG; ++u) {
-        if (g0 + u < sa.nseg) {
-          const int64_t i = (int64_t)(g0 + u) * sa.nbins + b;
-          sa.S[i] = 0;
-          sa.P[i] = run;
-          run += v[u];
-        }
+    for (int u = 0; u < kG; ++u) {
+      if (g0 + u < sa.nseg) {
+        const int64_t i = (int64_t)(g0 + u) * sa.nbins + b;
+        sa.S[i] = 0;
+        sa.P[i] = run;
+        run += v[u];
       }
     }
-    sa.bin_start[b] = run;
   }
-  __syncthreads();
+  sa.bin_start[b] = run;
+}
+
+// one block: bin totals (in bin_start) -> exclusive scan = bin_start; subproblems per bin ->
+// sub_start; bin_start[nbins] = M; nsub
+__device__ __forceinline__ void bin_scan(const ScanArgs& sa) {
   int carry = 0, carry_sub = 0;
   for (int base = 0; base < sa.nbins; base += blockDim.x) {
     const int b = base + threadIdx.x;
@@ -158,7 +163,37 @@ __global__ void __launch_bounds__(1024) k_seg_scan(ScanArgs sa) {
   }
 }
 
-// k_keys: a block of 64 W threads folds the points of chunk blockIdx.x
+// few bins: the segment prefixes and the bin scan in one block
+__global__ void __launch_bounds__(1024) k_seg_scan(ScanArgs sa) {
+  for (int b = threadIdx.x; b < sa.nbins; b += blockDim.x) seg_prefix_bin(sa, b);
+  __syncthreads();
+  bin_scan(sa);
+}
+
+// many bins: segment prefixes one thread per bin, then the bin scan in one block, then the
+// chunk rows of the histogram turned into first output slots, H[r][b] -> slot of (bin b,
+// chunk r), so that a k_scatter block reads one offset per bin
+__global__ void __launch_bounds__(256) k_seg_prefix(ScanArgs sa) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < sa.nbins) seg_prefix_bin(sa, b);
+}
+
+__global__ void __launch_bounds__(1024) k_bin_scan(ScanArgs sa) { bin_scan(sa); }
+
+__global__ void __launch_bounds__(256) k_row_offsets(ScanArgs sa) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int seg = blockIdx.y;
+  if (b >= sa.nbins) return;
+  const int r0 = seg * kRowSeg, r1 = min(sa.NB, r0 + kRowSeg);
+  int run = sa.bin_start[b] + sa.P[(int64_t)seg * sa.nbins + b];
+  for (int r = r0; r < r1; ++r) {
+    const int t = sa.H[(int64_t)r * sa.nbins + b];
+    sa.H[(int64_t)r * sa.nbins + b] = run;
+    run += t;
+  }
+}
+
+// k_keys: a block of 1024 threads folds the points of chunk blockIdx.x
 // ([lo, lo + CH)), stores their keys and counts them.
 template <typename CT, int D>
 __global__ void __launch_bounds__(1024) k_keys(KeyArgs ka, int64_t M, int CH, int nbins, int32_t* __restrict__ keys,
@@ -289,8 +324,13 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
   // segments + the rows of this segment before the chunk; then earlier waves' counts
   const int chunk = (int)blockIdx.x, seg = chunk / kRowSeg;
   for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-    int run = sa.bin_start[b] + sa.P[(int64_t)seg * nbins + b];
-    for (int r = seg * kRowSeg; r < chunk; ++r) run += sa.H[(int64_t)r * nbins + b];
+    int run;
+    if (sa.rows_are_offsets) {
+      run = sa.H[(int64_t)chunk * nbins + b];
+    } else {
+      run = sa.bin_start[b] + sa.P[(int64_t)seg * nbins + b];
+      for (int r = seg * kRowSeg; r < chunk; ++r) run += sa.H[(int64_t)r * nbins + b];
+    }
     for (int w = 0; w < W; ++w) {
       const int t = wcnt[w * nbins + b];
       wcnt[w * nbins + b] = run;
@@ -360,9 +400,22 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
   sa.maxsub = ps.maxsub;
   sa.max_subprob = ps.max_subprob;
   const size_t lds_keys = (size_t)nbins * sizeof(int32_t);
-  hipLaunchKernelGGL((k_keys<CT, D>), dim3(ps.NB), dim3(64 * W), lds_keys, s, ka, ps.M, CH, nbins, ps.keys, sa);
-  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sa);
+  // more than one bin per scatter thread: offsets of every (chunk, bin) in a pass of their own
+  sa.rows_are_offsets = nbins > 1024 ? 1 : 0;
+  hipLaunchKernelGGL((k_keys<CT, D>), dim3(ps.NB), dim3(1024), lds_keys, s, ka, ps.M, CH, nbins, ps.keys, sa);
+  if (sa.rows_are_offsets) {
+    hipLaunchKernelGGL(k_seg_prefix, dim3((nbins + 255) / 256), dim3(256), 0, s, sa);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, sa);
+    hipLaunchKernelGGL(k_row_offsets, dim3((nbins + 255) / 256, sa.nseg), dim3(256), 0, s, sa);
+  } else {
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sa);
+  }
   const size_t lds = (size_t)W * nbins * sizeof(int32_t);
+  if (lds > 65536) {
+    (void)hipFuncSetAttribute((const void*)k_scatter<CT, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)k_scatter<CT, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void*)k_scatter<CT, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  }
   if (D == 2 && ra.recs)
     hipLaunchKernelGGL((k_scatter<CT, 1>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
                        ps.perm, ra);
@@ -390,9 +443,9 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   }
   ps.maxsub = tune.maxsub;
   const int64_t M = ps.M;
-  // waves per chunk (up to 16): their LDS counters (W * nbins ints) must fit 64 KiB
+  // waves per chunk (up to 16): their LDS counters (W * nbins ints) must fit the device's LDS
   int W = 16;
-  while (W > 1 && (int64_t)W * nbins * (int64_t)sizeof(int32_t) > 65536) W /= 2;
+  while (W > 1 && (int64_t)W * nbins * (int64_t)sizeof(int32_t) > (int64_t)lds_device_max_bytes()) W /= 2;
   // points per wave: chunk / W, and enough that the histogram fits hist_cap
   int64_t SC = std::max<int64_t>(tune.chunk / W, 64);
   const int64_t need = (M * (int64_t)nbins + tune.hist_cap - 1) / tune.hist_cap;  // min points per chunk

@@ -43,7 +43,7 @@ def _need_gpu():
 
 
 @pytest.mark.parametrize("dim,n_modes,M", [(1, (1000,), 50000), (2, (100, 70), 30000), (2, (512, 512), 200000),
-                                           (3, (40, 32, 24), 40000)])
+                                           (3, (40, 32, 24), 40000), (3, (64, 64, 64), 200000)])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
 def test_bin_sort_bit_exact(rng, dim, n_modes, M, dist):
     from nufftax_amd._plan import F32, transform_plan
@@ -400,3 +400,45 @@ def test_interp2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
     plan.interp(_t(fw), c)
     ref = R.interp([x.astype(np.float64) for x in coords], fw.astype(np.complex128), kp)
     assert R.rel_l2(_np(c), ref) < 2e-6
+
+
+# ------------------------------------------------------------------ fused FFT + deconvolution
+
+
+@pytest.mark.parametrize("nm", [(63, 31), (256, 128), (512, 512)])
+@pytest.mark.parametrize("isign", [1, -1])
+@pytest.mark.parametrize("modeord", [0, 1])
+def test_fused_fft_matches_rocfft(rng, monkeypatch, nm, isign, modeord):
+    """Plans built with NUFFTX_FUSED_FFT run FFT + deconvolution of 2D FP32 power-of-two grids
+    as two fused passes; the same plans built without it take rocFFT + the separate
+    deconvolution kernels.  Type 1 and type 2, two transforms per call; relative l2 < 2e-6
+    (FP32 FFT rounding)."""
+    from nufftax_amd._plan import F32, Plan
+
+    M = 20000
+    x = _t(rng.uniform(-np.pi, np.pi, M).astype(np.float32))
+    y = _t(rng.uniform(-np.pi, np.pi, M).astype(np.float32))
+    c = _t((rng.standard_normal((2, M)) + 1j * rng.standard_normal((2, M))).astype(np.complex64))
+    f = _t((rng.standard_normal((2, nm[1], nm[0])) + 1j * rng.standard_normal((2, nm[1], nm[0]))).astype(np.complex64))
+
+    def run(rocfft):
+        if rocfft:
+            monkeypatch.delenv("NUFFTX_FUSED_FFT", raising=False)
+        else:
+            monkeypatch.setenv("NUFFTX_FUSED_FFT", "1")
+        p1 = Plan.transform(1, 2, nm, 2, 1e-6, isign, 2.0, F32, modeord, torch.device(DEV))
+        p2 = Plan.transform(2, 2, nm, 2, 1e-6, -isign, 2.0, F32, modeord, torch.device(DEV))
+        assert p1.grid_size(0) == 2 ** int(np.log2(p1.grid_size(0)))
+        p1.setpts([x, y])
+        p2.setpts([x, y])
+        o1 = torch.empty((2, nm[1], nm[0]), dtype=torch.complex64, device=DEV)
+        o2 = torch.empty((2, M), dtype=torch.complex64, device=DEV)
+        p1.execute(c, o1)
+        p2.execute(o2, f)
+        torch.cuda.synchronize()
+        return _np(o1), _np(o2)
+
+    a1, a2 = run(False)
+    b1, b2 = run(True)
+    assert R.rel_l2(a1, b1) < 2e-6
+    assert R.rel_l2(a2, b2) < 2e-6
