@@ -10,13 +10,17 @@
 //            columns: mid[t][m0][y] -> f[t][m1][m0]  = fac1[m1] * sum_y mid e(k1(m1) y / nf1)
 //   type 2   columns: f[t][m1][m0]  -> mid[t][m0][y] = sum_m1 fac0 fac1 f e(k1(m1) y / nf1)
 //            rows:    mid[t][m0][y] -> grid[t][y][x] = sum_m0 mid e(k0(m0) x / nf0)
-// with e(u) = exp(isign 2 pi i u) and k(m) the FFT bin of mode index m (mode_to_grid).  The
-// intermediate mid holds only nm0 of the nf0 x-bins, transposed so that both passes read and
-// write whole sequences; a block handles R = 4 sequences (32-byte runs on the transposed side).
+// with e(u) = exp(isign 2 pi i u) and k(m) the FFT bin of mode index m.  The intermediate mid
+// holds only nm0 of the nf0 x-bins, transposed so that both passes read and write whole
+// sequences; a block handles S sequences (S 8-byte runs on the transposed side).
 //
-// Each sequence is transformed in LDS by an in-place radix-2 decimation-in-frequency FFT
-// (results in bit-reversed order); twiddles exp(-2 pi i k / N), k < N/2, come from a table
-// computed on the host in FP64 and are conjugated for isign > 0.
+// Each sequence of N = R1 R2 points is transformed in LDS by the four-step algorithm: R1
+// register DFTs of R2 points over the stride-R1 subsequences, the twiddles W_N^(n1 k2), then
+// R2 register DFTs of R1 points over contiguous runs, written back in natural order
+// (X[k2 + R2 k1]).  A register DFT is a radix-2 decimation-in-time FFT on bit-reversed
+// loads.  Twiddles exp(-2 pi i k / N), k < N/2, come from a table computed on the host in
+// FP64 (staged in LDS) and are conjugated for isign > 0.  A block of kFftThreads threads
+// transforms S = kFftThreads / R1 sequences (R1 >= R2).
 #include <cmath>
 #include <vector>
 
@@ -26,8 +30,7 @@ namespace nufftx {
 
 namespace {
 
-constexpr int kFftSeq = 4;         // sequences per block
-constexpr int kFftThreads = 256;
+constexpr int kFftThreads = 128;
 
 __device__ __forceinline__ int fft_mode_to_bin(int m, int n, int nf, int modeord) {
   const int kmin = -(n / 2), kmax = (n - 1) / 2;
@@ -35,33 +38,105 @@ __device__ __forceinline__ int fft_mode_to_bin(int m, int n, int nf, int modeord
   return k >= 0 ? k : nf + k;
 }
 
-__device__ __forceinline__ int bitrev(int k, int logn) { return (int)(__brev((unsigned)k) >> (32 - logn)); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
 
-// In-place DIF FFT of kFftSeq sequences of length N = 2^logn at buf[r * N + n]; twiddle table
-// tw[k] = exp(-2 pi i k / N) in LDS.  Ends with a barrier.
-__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, int logn, bool conj) {
-  const int N = 1 << logn;
-  const int nbf = kFftSeq * (N >> 1);
-  for (int lh = logn - 1; lh >= 0; --lh) {
-    const int half = 1 << lh;
-    const int tstep = (N >> 1) >> lh;  // N / (2 half)
-    for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
-      const int r = t >> (logn - 1), bf = t & ((N >> 1) - 1);
-      const int pos = bf & (half - 1);
-      const int i = r * N + ((bf >> lh) << (lh + 1)) + pos, j = i + half;
-      const float2 a = buf[i], b = buf[j];
-      float2 w = tw[pos * tstep];
-      if (conj) w.y = -w.y;
-      const float dr = a.x - b.x, di = a.y - b.y;
-      buf[i] = make_float2(a.x + b.x, a.y + b.y);
-      buf[j] = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
+// W_N^m for 0 <= m < N from the half table tw[k] = exp(-2 pi i k / N), conjugated if conj
+__device__ __forceinline__ float2 twiddle(const float2* tw, int m, int N, bool conj) {
+  const int h = N >> 1;
+  float2 w = m < h ? tw[m] : make_float2(-tw[m - h].x, -tw[m - h].y);
+  if (conj) w.y = -w.y;
+  return w;
+}
+
+template <int L>
+__host__ __device__ constexpr int bitrev_c(int i) {
+  int r = 0;
+  for (int b = 0; b < L; ++b) r |= ((i >> b) & 1) << (L - 1 - b);
+  return r;
+}
+
+// LDS layout of a sequence of N = 2^LOGN points: one pad slot after every R1 points, so that
+// the contiguous runs read in step B of fft_4step start on different banks
+template <int LOGN>
+struct FftLayout {
+  static constexpr int L1 = (LOGN + 1) / 2;
+  static constexpr int N = 1 << LOGN;
+  static constexpr int PL = N + (N >> L1);  // padded sequence length
+  static __device__ __forceinline__ int at(int r, int n) { return r * PL + n + (n >> L1); }
+};
+
+// In-register DFT of R = 2^L points held in bit-reversed order (v[bitrev(n)] = x[n]); result
+// in natural order.  W_R^j = W_N^(j N / R).
+template <int L>
+__device__ __forceinline__ void reg_dft(float2 (&v)[1 << L], const float2* tw, int N, bool conj) {
+  constexpr int R = 1 << L;
+  const int step = N / R;
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1) {
+    const int half = len >> 1;
+#pragma unroll
+    for (int j = 0; j < half; ++j) {
+      const float2 w = twiddle(tw, j * (R / len) * step, N, conj);
+#pragma unroll
+      for (int i = 0; i < R; i += len) {
+        const float2 a = v[i + j], b = cmul(v[i + j + half], w);
+        v[i + j] = make_float2(a.x + b.x, a.y + b.y);
+        v[i + j + half] = make_float2(a.x - b.x, a.y - b.y);
+      }
     }
-    __syncthreads();
   }
 }
 
+// In-place FFT of S = kFftThreads / R1 sequences of N = 2^(L1 + L2) points in the padded
+// layout FftLayout (natural order in and out).  Ends with a barrier.
+template <int L1, int L2>
+__device__ __forceinline__ void fft_4step(float2* buf, const float2* tw, bool conj) {
+  constexpr int R1 = 1 << L1, R2 = 1 << L2, N = R1 * R2;
+  constexpr int S = kFftThreads / R1;
+  using Lay = FftLayout<L1 + L2>;
+  // step A: thread (r, n1) transforms x[n1 + R1 n2] over n2 and scales by W_N^(n1 k2)
+  {
+    const int r = threadIdx.x / R1, n1 = threadIdx.x % R1;
+    float2 v[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) v[bitrev_c<L2>(n2)] = buf[Lay::at(r, n1 + R1 * n2)];
+    reg_dft<L2>(v, tw, N, conj);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2)
+      buf[Lay::at(r, n1 + R1 * k2)] = k2 == 0 ? v[0] : cmul(v[k2], twiddle(tw, n1 * k2, N, conj));
+  }
+  __syncthreads();
+  // step B: thread (r, k2) transforms y[n1 + R1 k2] over n1 into X[k2 + R2 k1]; every read
+  // precedes every write (the runs read and the slots written belong to different threads)
+  float2 v[R1];
+  const bool act = (int)threadIdx.x < S * R2;
+  const int r = threadIdx.x / R2, k2 = threadIdx.x % R2;
+  if (act) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) v[bitrev_c<L1>(n1)] = buf[Lay::at(r, n1 + R1 * k2)];
+    reg_dft<L1>(v, tw, N, conj);
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[Lay::at(r, k2 + R2 * k1)] = v[k1];
+  }
+  __syncthreads();
+}
+
+// N = 2^LOGN, 6 <= LOGN <= 12: R1 = 2^ceil(LOGN / 2), R2 = 2^floor(LOGN / 2)
+template <int LOGN>
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, bool conj) {
+  fft_4step<(LOGN + 1) / 2, LOGN / 2>(buf, tw, conj);
+}
+
+// sequences per block: kFftThreads / R1
+__host__ __device__ constexpr int fft_seqs(int logn) { return kFftThreads >> ((logn + 1) / 2); }
+
 __device__ __forceinline__ void load_twiddles(float2* stw, const float2* __restrict__ tw, int n) {
-  for (int k = threadIdx.x; k < (n >> 1); k += blockDim.x) stw[k] = tw[k];
+  for (int k = threadIdx.x; k < (n >> 1); k += kFftThreads) stw[k] = tw[k];
 }
 
 struct Fft2K {
@@ -73,113 +148,141 @@ struct Fft2K {
   int64_t grid_cells, mode_cells, mid_cells;  // per transform
 };
 
-// type 1, rows: grid rows y0 .. y0+R-1 -> mid[m0][y] (kept x-modes, times fac0)
+// type 1, rows: grid rows y0 .. y0+S-1 -> mid[m0][y] (kept x-modes, times fac0)
+template <int LOGN>
 __global__ void __launch_bounds__(kFftThreads) k_fft2_rows_t1(Fft2K k, const float2* __restrict__ grid,
                                                               float2* __restrict__ mid, int conj) {
   extern __shared__ __align__(16) float2 smem[];
-  const int N = k.nf[0], ny = k.nf[1];
+  constexpr int N = 1 << LOGN, S = fft_seqs(LOGN);
+  const int ny = k.nf[1];
   float2* stw = smem;
   float2* buf = smem + (N >> 1);
-  const int y0 = blockIdx.x * kFftSeq;
+  const int y0 = blockIdx.x * S;
   const int64_t t = blockIdx.y;
   const float2* g = grid + t * k.grid_cells;
   load_twiddles(stw, k.tw[0], N);
-  for (int e = threadIdx.x; e < kFftSeq * N; e += blockDim.x) {
+#pragma unroll
+  for (int u = 0; u < S * N / kFftThreads; ++u) {
+    const int e = threadIdx.x + u * kFftThreads;
     const int r = e / N, x = e - r * N;
-    buf[e] = y0 + r < ny ? g[(int64_t)(y0 + r) * N + x] : make_float2(0.f, 0.f);
+    buf[FftLayout<LOGN>::at(r, x)] = y0 + r < ny ? g[(int64_t)(y0 + r) * N + x] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft(buf, stw, k.logn[0], conj != 0);
+  lds_fft<LOGN>(buf, stw, conj != 0);
   float2* o = mid + t * k.mid_cells;
-  for (int e = threadIdx.x; e < kFftSeq * k.nm[0]; e += blockDim.x) {
-    const int m0 = e / kFftSeq, r = e - m0 * kFftSeq;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < S * k.nm[0]; e += kFftThreads) {
+    const int m0 = e / S, r = e - m0 * S;
     if (y0 + r >= ny) continue;
-    const float2 v = buf[r * N + bitrev(fft_mode_to_bin(m0, k.nm[0], N, k.modeord), k.logn[0])];
+    const float2 v = buf[FftLayout<LOGN>::at(r, fft_mode_to_bin(m0, k.nm[0], N, k.modeord))];
     const float f = k.fac[0][m0];
     o[(int64_t)m0 * ny + y0 + r] = make_float2(v.x * f, v.y * f);
   }
 }
 
-// type 1, columns: mid[m0 .. m0+R-1][y] -> f[m1][m0] (kept y-modes, times fac1)
+// type 1, columns: mid[m0 .. m0+S-1][y] -> f[m1][m0] (kept y-modes, times fac1)
+template <int LOGN>
 __global__ void __launch_bounds__(kFftThreads) k_fft2_cols_t1(Fft2K k, const float2* __restrict__ mid,
                                                               float2* __restrict__ f, int conj) {
   extern __shared__ __align__(16) float2 smem[];
-  const int N = k.nf[1], nm0 = k.nm[0];
+  constexpr int N = 1 << LOGN, S = fft_seqs(LOGN);
+  const int nm0 = k.nm[0];
   float2* stw = smem;
   float2* buf = smem + (N >> 1);
-  const int c0 = blockIdx.x * kFftSeq;
+  const int c0 = blockIdx.x * S;
   const int64_t t = blockIdx.y;
   const float2* in = mid + t * k.mid_cells;
   load_twiddles(stw, k.tw[1], N);
-  for (int e = threadIdx.x; e < kFftSeq * N; e += blockDim.x) {
+#pragma unroll
+  for (int u = 0; u < S * N / kFftThreads; ++u) {
+    const int e = threadIdx.x + u * kFftThreads;
     const int r = e / N, y = e - r * N;
-    buf[e] = c0 + r < nm0 ? in[(int64_t)(c0 + r) * N + y] : make_float2(0.f, 0.f);
+    buf[FftLayout<LOGN>::at(r, y)] = c0 + r < nm0 ? in[(int64_t)(c0 + r) * N + y] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft(buf, stw, k.logn[1], conj != 0);
+  lds_fft<LOGN>(buf, stw, conj != 0);
   float2* o = f + t * k.mode_cells;
-  for (int e = threadIdx.x; e < kFftSeq * k.nm[1]; e += blockDim.x) {
-    const int m1 = e / kFftSeq, r = e - m1 * kFftSeq;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < S * k.nm[1]; e += kFftThreads) {
+    const int m1 = e / S, r = e - m1 * S;
     if (c0 + r >= nm0) continue;
-    const float2 v = buf[r * N + bitrev(fft_mode_to_bin(m1, k.nm[1], N, k.modeord), k.logn[1])];
+    const float2 v = buf[FftLayout<LOGN>::at(r, fft_mode_to_bin(m1, k.nm[1], N, k.modeord))];
     const float fa = k.fac[1][m1];
     o[(int64_t)m1 * nm0 + c0 + r] = make_float2(v.x * fa, v.y * fa);
   }
 }
 
-// type 2, columns: f[m1][m0 .. m0+R-1] placed at the y-bins (times fac0 fac1) -> mid[m0][y]
+// type 2, columns: f[m1][m0 .. m0+S-1] placed at the y-bins (times fac0 fac1) -> mid[m0][y]
+template <int LOGN>
 __global__ void __launch_bounds__(kFftThreads) k_fft2_cols_t2(Fft2K k, const float2* __restrict__ f,
                                                               float2* __restrict__ mid, int conj) {
   extern __shared__ __align__(16) float2 smem[];
-  const int N = k.nf[1], nm0 = k.nm[0];
+  constexpr int N = 1 << LOGN, S = fft_seqs(LOGN);
+  const int nm0 = k.nm[0];
   float2* stw = smem;
   float2* buf = smem + (N >> 1);
-  const int c0 = blockIdx.x * kFftSeq;
+  const int c0 = blockIdx.x * S;
   const int64_t t = blockIdx.y;
   const float2* in = f + t * k.mode_cells;
   load_twiddles(stw, k.tw[1], N);
-  for (int e = threadIdx.x; e < kFftSeq * N; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int u = 0; u < S * FftLayout<LOGN>::PL / kFftThreads + 1; ++u) {
+    const int e = threadIdx.x + u * kFftThreads;
+    if (e < S * FftLayout<LOGN>::PL) buf[e] = make_float2(0.f, 0.f);
+  }
   __syncthreads();
-  for (int e = threadIdx.x; e < kFftSeq * k.nm[1]; e += blockDim.x) {
-    const int m1 = e / kFftSeq, r = e - m1 * kFftSeq;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < S * k.nm[1]; e += kFftThreads) {
+    const int m1 = e / S, r = e - m1 * S;
     if (c0 + r >= nm0) continue;
     const float2 v = in[(int64_t)m1 * nm0 + c0 + r];
     const float fa = k.fac[0][c0 + r] * k.fac[1][m1];
-    buf[r * N + fft_mode_to_bin(m1, k.nm[1], N, k.modeord)] = make_float2(v.x * fa, v.y * fa);
+    buf[FftLayout<LOGN>::at(r, fft_mode_to_bin(m1, k.nm[1], N, k.modeord))] = make_float2(v.x * fa, v.y * fa);
   }
   __syncthreads();
-  lds_fft(buf, stw, k.logn[1], conj != 0);
+  lds_fft<LOGN>(buf, stw, conj != 0);
   float2* o = mid + t * k.mid_cells;
-  for (int e = threadIdx.x; e < kFftSeq * N; e += blockDim.x) {
+#pragma unroll
+  for (int u = 0; u < S * N / kFftThreads; ++u) {
+    const int e = threadIdx.x + u * kFftThreads;
     const int r = e / N, y = e - r * N;
-    if (c0 + r < nm0) o[(int64_t)(c0 + r) * N + y] = buf[r * N + bitrev(y, k.logn[1])];
+    if (c0 + r < nm0) o[(int64_t)(c0 + r) * N + y] = buf[FftLayout<LOGN>::at(r, y)];
   }
 }
 
-// type 2, rows: mid[m0][y0 .. y0+R-1] placed at the x-bins -> grid rows y0 .. y0+R-1
+// type 2, rows: mid[m0][y0 .. y0+S-1] placed at the x-bins -> grid rows y0 .. y0+S-1
+template <int LOGN>
 __global__ void __launch_bounds__(kFftThreads) k_fft2_rows_t2(Fft2K k, const float2* __restrict__ mid,
                                                               float2* __restrict__ grid, int conj) {
   extern __shared__ __align__(16) float2 smem[];
-  const int N = k.nf[0], ny = k.nf[1];
+  constexpr int N = 1 << LOGN, S = fft_seqs(LOGN);
+  const int ny = k.nf[1];
   float2* stw = smem;
   float2* buf = smem + (N >> 1);
-  const int y0 = blockIdx.x * kFftSeq;
+  const int y0 = blockIdx.x * S;
   const int64_t t = blockIdx.y;
   const float2* in = mid + t * k.mid_cells;
   load_twiddles(stw, k.tw[0], N);
-  for (int e = threadIdx.x; e < kFftSeq * N; e += blockDim.x) buf[e] = make_float2(0.f, 0.f);
-  __syncthreads();
-  for (int e = threadIdx.x; e < kFftSeq * k.nm[0]; e += blockDim.x) {
-    const int m0 = e / kFftSeq, r = e - m0 * kFftSeq;
-    if (y0 + r >= ny) continue;
-    buf[r * N + fft_mode_to_bin(m0, k.nm[0], N, k.modeord)] = in[(int64_t)m0 * ny + y0 + r];
+#pragma unroll
+  for (int u = 0; u < S * FftLayout<LOGN>::PL / kFftThreads + 1; ++u) {
+    const int e = threadIdx.x + u * kFftThreads;
+    if (e < S * FftLayout<LOGN>::PL) buf[e] = make_float2(0.f, 0.f);
   }
   __syncthreads();
-  lds_fft(buf, stw, k.logn[0], conj != 0);
+#pragma unroll 4
+  for (int e = threadIdx.x; e < S * k.nm[0]; e += kFftThreads) {
+    const int m0 = e / S, r = e - m0 * S;
+    if (y0 + r >= ny) continue;
+    buf[FftLayout<LOGN>::at(r, fft_mode_to_bin(m0, k.nm[0], N, k.modeord))] = in[(int64_t)m0 * ny + y0 + r];
+  }
+  __syncthreads();
+  lds_fft<LOGN>(buf, stw, conj != 0);
   float2* g = grid + t * k.grid_cells;
-  for (int e = threadIdx.x; e < kFftSeq * N; e += blockDim.x) {
+#pragma unroll
+  for (int u = 0; u < S * N / kFftThreads; ++u) {
+    const int e = threadIdx.x + u * kFftThreads;
     const int r = e / N, x = e - r * N;
-    if (y0 + r < ny) g[(int64_t)(y0 + r) * N + x] = buf[r * N + bitrev(x, k.logn[0])];
+    if (y0 + r < ny) g[(int64_t)(y0 + r) * N + x] = buf[FftLayout<LOGN>::at(r, x)];
   }
 }
 
@@ -205,11 +308,19 @@ Fft2K make_k(const Fft2Plan& fp, const DeconvArgs& a) {
   return k;
 }
 
-size_t fft_lds(int n) { return (size_t)(n / 2 + kFftSeq * n) * sizeof(float2); }
+// LDS of a pass over length-n sequences (twiddles + S sequences)
+size_t fft_lds(int n) {
+  const int l = log2_exact(n);
+  const int pl = n + (n >> ((l + 1) / 2));
+  return (size_t)(n / 2 + fft_seqs(l) * pl) * sizeof(float2);
+}
 
-// LDS of a pass over length-n sequences; grids of 2048 and 4096 need more than 64 KB
-void fft_lds_attr(const void* kern, int n) {
-  const size_t b = fft_lds(n);
+unsigned fft_blocks(int count, int n) {
+  const int S = fft_seqs(log2_exact(n));
+  return (unsigned)((count + S - 1) / S);
+}
+
+void fft_lds_attr(const void* kern, size_t b) {
   if (b > 65536) (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
 }
 
@@ -256,18 +367,45 @@ void fft2_plan_free(Fft2Plan& fp) {
   fp.cap_mid = 0;
 }
 
+// launch kern<LOGN> for the runtime log2 of the transformed axis
+template <template <int> class K, typename... Args>
+void launch_logn(int logn, dim3 grid, size_t lds, hipStream_t s, Args... args) {
+  switch (logn) {
+    case 6: K<6>::launch(grid, lds, s, args...); break;
+    case 7: K<7>::launch(grid, lds, s, args...); break;
+    case 8: K<8>::launch(grid, lds, s, args...); break;
+    case 9: K<9>::launch(grid, lds, s, args...); break;
+    case 10: K<10>::launch(grid, lds, s, args...); break;
+    case 11: K<11>::launch(grid, lds, s, args...); break;
+    default: K<12>::launch(grid, lds, s, args...); break;
+  }
+}
+
+#define FFT2_LAUNCHER(name)                                                                    \
+  template <int L>                                                                             \
+  struct name##_l {                                                                            \
+    template <typename... Args>                                                                \
+    static void launch(dim3 grid, size_t lds, hipStream_t s, Args... args) {                   \
+      fft_lds_attr((const void*)name<L>, lds);                                                 \
+      hipLaunchKernelGGL(name<L>, grid, dim3(kFftThreads), lds, s, args...);                   \
+    }                                                                                          \
+  };
+FFT2_LAUNCHER(k_fft2_rows_t1)
+FFT2_LAUNCHER(k_fft2_cols_t1)
+FFT2_LAUNCHER(k_fft2_cols_t2)
+FFT2_LAUNCHER(k_fft2_rows_t2)
+#undef FFT2_LAUNCHER
+
 hipError_t fft2_shuffle(Fft2Plan& fp, const DeconvArgs& a, const void* grid, void* f, int isign, hipStream_t s) {
   const Fft2K k = make_k(fp, a);
   hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * k.mid_cells, fp.retired);
   if (e != hipSuccess) return e;
   const int conj = isign > 0 ? 1 : 0;
   const unsigned nt = (unsigned)a.n_trans;
-  fft_lds_attr((const void*)k_fft2_rows_t1, a.nf[0]);
-  fft_lds_attr((const void*)k_fft2_cols_t1, a.nf[1]);
-  hipLaunchKernelGGL(k_fft2_rows_t1, dim3((a.nf[1] + kFftSeq - 1) / kFftSeq, nt), dim3(kFftThreads), fft_lds(a.nf[0]),
-                     s, k, static_cast<const float2*>(grid), fp.mid, conj);
-  hipLaunchKernelGGL(k_fft2_cols_t1, dim3((a.nm[0] + kFftSeq - 1) / kFftSeq, nt), dim3(kFftThreads), fft_lds(a.nf[1]),
-                     s, k, fp.mid, static_cast<float2*>(f), conj);
+  launch_logn<k_fft2_rows_t1_l>(k.logn[0], dim3(fft_blocks(a.nf[1], a.nf[0]), nt), fft_lds(a.nf[0]), s, k,
+                                static_cast<const float2*>(grid), fp.mid, conj);
+  launch_logn<k_fft2_cols_t1_l>(k.logn[1], dim3(fft_blocks(a.nm[0], a.nf[1]), nt), fft_lds(a.nf[1]), s, k,
+                                (const float2*)fp.mid, static_cast<float2*>(f), conj);
   return hipGetLastError();
 }
 
@@ -277,12 +415,10 @@ hipError_t fft2_pad(Fft2Plan& fp, const DeconvArgs& a, const void* f, void* grid
   if (e != hipSuccess) return e;
   const int conj = isign > 0 ? 1 : 0;
   const unsigned nt = (unsigned)a.n_trans;
-  fft_lds_attr((const void*)k_fft2_cols_t2, a.nf[1]);
-  fft_lds_attr((const void*)k_fft2_rows_t2, a.nf[0]);
-  hipLaunchKernelGGL(k_fft2_cols_t2, dim3((a.nm[0] + kFftSeq - 1) / kFftSeq, nt), dim3(kFftThreads), fft_lds(a.nf[1]),
-                     s, k, static_cast<const float2*>(f), fp.mid, conj);
-  hipLaunchKernelGGL(k_fft2_rows_t2, dim3((a.nf[1] + kFftSeq - 1) / kFftSeq, nt), dim3(kFftThreads), fft_lds(a.nf[0]),
-                     s, k, fp.mid, static_cast<float2*>(grid), conj);
+  launch_logn<k_fft2_cols_t2_l>(k.logn[1], dim3(fft_blocks(a.nm[0], a.nf[1]), nt), fft_lds(a.nf[1]), s, k,
+                                static_cast<const float2*>(f), fp.mid, conj);
+  launch_logn<k_fft2_rows_t2_l>(k.logn[0], dim3(fft_blocks(a.nf[1], a.nf[0]), nt), fft_lds(a.nf[0]), s, k,
+                                (const float2*)fp.mid, static_cast<float2*>(grid), conj);
   return hipGetLastError();
 }
 

@@ -309,7 +309,8 @@ struct nufftx_plan_s {
   void* fac_in[3] = {nullptr, nullptr, nullptr};
   FftPlan fft, fft_in;
   Fft2Plan fft2;          // fused FFT + deconvolution (2D FP32, power-of-two grids)
-  bool use_fft2 = false;
+  bool fft2_pad = false;      // modes -> grid (type 2 input, adjoint of type 1) fused
+  bool fft2_shuffle = false;  // grid -> modes (type 1 output) fused
   QuadRule quad;
   // type 3
   T3State* t3 = nullptr;
@@ -399,7 +400,7 @@ DeconvArgs dargs(nufftx_plan p, bool inner) {
 
 // fine grid -> FFT -> deconvolved modes f (type 1 output stage)
 int grid_to_modes(nufftx_plan p, void* f, int isign) {
-  if (p->use_fft2) {
+  if (p->fft2_shuffle) {
     LAUNCH(fft2_shuffle(p->fft2, dargs(p, false), p->grid, f, isign, p->stream));
     return NUFFTX_OK;
   }
@@ -410,7 +411,7 @@ int grid_to_modes(nufftx_plan p, void* f, int isign) {
 
 // modes f -> deconvolved, padded -> FFT -> fine grid (type 2 input stage)
 int modes_to_grid(nufftx_plan p, const void* f, int isign) {
-  if (p->use_fft2) {
+  if (p->fft2_pad) {
     LAUNCH(fft2_pad(p->fft2, dargs(p, false), f, p->grid, isign, p->stream));
     return NUFFTX_OK;
   }
@@ -531,15 +532,17 @@ int nufftx_makeplan(int type, int dim, const int64_t* n_modes, int isign, int64_
       if ((r = upload_factors(p->fac[d], deconv_factors(ph, p->n_modes[d], modeord), p->prec))) goto fail_out;
     }
     if ((r = fft_make(p->fft, dim, p->g.nf, n_trans, p->prec))) goto fail_out;
-    // NUFFTX_FUSED_FFT selects the fused radix-2 FFT + deconvolution passes; measured
-    // slower than rocFFT + the deconvolution kernel on 1024^2 (42 against 32 us per
-    // transform), so rocFFT stays the default
-    if (fft2_applicable(dargs(p, false)) && std::getenv("NUFFTX_FUSED_FFT")) {
+    // Fused FFT + deconvolution passes on 1024^2 (MI355X): modes -> grid 27 us against 33 us
+    // for rocFFT + the padding kernel, grid -> modes 39 us against 31 us for rocFFT + the
+    // shuffle kernel.  So the pad direction is fused by default; NUFFTX_FUSED_FFT fuses both
+    // directions, NUFFTX_ROCFFT neither.
+    if (fft2_applicable(dargs(p, false)) && !std::getenv("NUFFTX_ROCFFT")) {
       if (fft2_plan_init(p->fft2, p->g.nf) != hipSuccess) {
         r = fail(NUFFTX_ERR_ALLOC, "fused FFT twiddle tables");
         goto fail_out;
       }
-      p->use_fft2 = true;
+      p->fft2_pad = true;
+      p->fft2_shuffle = std::getenv("NUFFTX_FUSED_FFT") != nullptr;
     }
   } else {
     char* gi = nullptr;

@@ -410,9 +410,9 @@ def test_interp2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
 @pytest.mark.parametrize("modeord", [0, 1])
 def test_fused_fft_matches_rocfft(rng, monkeypatch, nm, isign, modeord):
     """Plans built with NUFFTX_FUSED_FFT run FFT + deconvolution of 2D FP32 power-of-two grids
-    as two fused passes; the same plans built without it take rocFFT + the separate
-    deconvolution kernels.  Type 1 and type 2, two transforms per call; relative l2 < 2e-6
-    (FP32 FFT rounding)."""
+    as two fused passes in both directions; the same plans built with NUFFTX_ROCFFT take
+    rocFFT + the separate deconvolution kernels.  Type 1 and type 2, two transforms per call;
+    relative l2 < 2e-6 (FP32 FFT rounding)."""
     from nufftax_amd._plan import F32, Plan
 
     M = 20000
@@ -424,7 +424,9 @@ def test_fused_fft_matches_rocfft(rng, monkeypatch, nm, isign, modeord):
     def run(rocfft):
         if rocfft:
             monkeypatch.delenv("NUFFTX_FUSED_FFT", raising=False)
+            monkeypatch.setenv("NUFFTX_ROCFFT", "1")
         else:
+            monkeypatch.delenv("NUFFTX_ROCFFT", raising=False)
             monkeypatch.setenv("NUFFTX_FUSED_FFT", "1")
         p1 = Plan.transform(1, 2, nm, 2, 1e-6, isign, 2.0, F32, modeord, torch.device(DEV))
         p2 = Plan.transform(2, 2, nm, 2, 1e-6, -isign, 2.0, F32, modeord, torch.device(DEV))
