@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # the same counters read for a 256 MiB copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
 # the summary with the commands is profiles/nufft2d_radial_4M_hbm_traffic.md.
 TRAFFIC_BYTES = {
-    "k_spread_banked<float,2,8,false,16>": 501375.9 * 1024 * 2.0 + 78768.4 * 1024,      # 1.11 GB
-    "k_gather_tiled<float,float,2,8,false,true>": 51747.6 * 1024 * 2.0 + 123608.3 * 1024,  # 0.23 GB
+    "k_spread_banked<float,2,8,false,16>": 500970.6 * 1024 * 2.0 + 78768.4 * 1024,      # 1.11 GB
+    "k_gather_tiled<float,float,2,8,false,true>": 51764.0 * 1024 * 2.0 + 123569.1 * 1024,  # 0.23 GB
 }
 
 
