@@ -103,6 +103,18 @@ int nufftx_execute(nufftx_plan plan, void* c, void* f);
 int nufftx_execute_sign(nufftx_plan plan, void* c, void* f, int isign);
 
 /*
+ * Execute count plans (all type 1 or all type 2, each with its own points, grid size and
+ * isign): plan i runs c[i] -> f[i] (type 1) or f[i] -> c[i] (type 2), as nufftx_execute.
+ * No counterpart in the reference (its nearest relative is the n_trans batching of
+ * nufftax/core/spread.py:100-173); for serving loops that batch requests of uneven M and
+ * grid size.  Up to 8 FP32 2D/3D plans with the same kernel width, tile shape, n_trans and
+ * stream share one spreading (type 1) or interpolation (type 2) launch; other batches run plan
+ * by plan.
+ * The plans must be distinct.
+ */
+int nufftx_execute_many(int count, const nufftx_plan* plans, void* const* c, void* const* f);
+
+/*
  * Reverse-mode derivative of a type-1 or type-2 plan (PyTorch cotangent
  * convention: cotangent = dL/dRe + i dL/dIm).  Replaces the JVP + transpose
  * rules of nufftax/transforms/primitives.py:88-110 (1D type 1), :127-151 (1D type 2),

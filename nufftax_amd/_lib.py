@@ -44,6 +44,7 @@ SIGNATURES = {
     "nufftx_setpts": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp]),
     "nufftx_execute": (ctypes.c_int, [_vp, _vp, _vp]),
     "nufftx_execute_sign": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int]),
+    "nufftx_execute_many": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "nufftx_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "nufftx_destroy": (ctypes.c_int, [_vp]),
     "nufftx_makeplan_grid": (

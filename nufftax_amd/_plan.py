@@ -105,6 +105,19 @@ class Plan:
         else:
             check(lib().nufftx_execute_sign(self.h, ptr(c), ptr(f), int(isign)), "nufftx_execute_sign")
 
+    @staticmethod
+    def execute_many(plans, cs, fs):
+        """Run plans[i] on (cs[i], fs[i]) as execute() does, sharing the spreading /
+        interpolation launch across the plans where the library can (nufftx_execute_many)."""
+        n = len(plans)
+        if n == 0:
+            return
+        for p in plans:
+            p._sync_stream()
+        arr = ctypes.c_void_p * n
+        check(lib().nufftx_execute_many(n, arr(*[p.h for p in plans]), arr(*[ptr(c) for c in cs]),
+                                        arr(*[ptr(f) for f in fs])), "nufftx_execute_many")
+
     def vjp(self, src, g, grad_src, grads):
         self._sync_stream()
         gs = list(grads) + [None] * (3 - len(grads))

@@ -118,6 +118,11 @@ struct GridOpArgs {
 hipError_t launch_spread(const GridOpArgs& a, const void* c, void* fw, hipStream_t s);
 // c[n_trans][M] = interp(fw)
 hipError_t launch_interp(const GridOpArgs& a, const void* fw, void* c, hipStream_t s);
+constexpr int kMaxManyPlans = 8;
+// The same for up to kMaxManyPlans point sets in one launch (FP32, 2D/3D, one kernel size and n_trans;
+// points and grids may differ).  hipErrorNotSupported: launch them one by one instead.
+hipError_t launch_spread_many(const GridOpArgs* a, int count, const void* const* c, void* const* fw, hipStream_t s);
+hipError_t launch_interp_many(const GridOpArgs* a, int count, const void* const* fw, void* const* c, hipStream_t s);
 // Fused gather with kernel-derivative weights.  For each point i and transform t:
 //   S0 = sum G w, S_ax = sum G dW/d ax   (dW/d ax includes the -nf/2pi chain factor)
 //   gval[t][i] = S0                               (if gval != nullptr)

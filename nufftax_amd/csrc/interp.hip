@@ -1,4 +1,5 @@
 // Type-2 interpolation launchers (kernels in si_kernels.hpp and si_mfma.hpp).
+#include <algorithm>
 #include <cstdlib>
 
 #include "si_mfma.hpp"
@@ -74,7 +75,55 @@ hipError_t interp_dim(const GridOpArgs& ga, const void* fw, void* c, hipStream_t
   return hipErrorInvalidValue;
 }
 
+template <typename CT, int D>
+hipError_t interp_many_impl(const GridOpArgs* gas, int count, const void* const* fw, void* const* c, hipStream_t s) {
+  size_t lmax = 0;
+  GatherMany m;
+  m.count = count;
+  m.first[0] = 0;
+  for (int i = 0; i < count; ++i) {
+    size_t l = 0;
+    if (!use_tiles(*gas[i].g, gas[i].prec, l)) return hipErrorNotSupported;
+    lmax = std::max(lmax, l);
+    m.a[i] = make_si_args<float>(gas[i]);
+    m.fw[i] = static_cast<const cplx<float>*>(fw[i]);
+    m.out[i] = static_cast<cplx<float>*>(c[i]);
+    m.first[i + 1] = m.first[i] + std::max(0, gas[i].ps->max_subprob);
+  }
+  for (int i = count; i < kMaxMany; ++i) m.first[i + 1] = m.first[count];
+  if (m.first[count] == 0) return hipSuccess;
+  const bool rec = m.a[0].crec != nullptr;
+  for (int i = 1; i < count; ++i)
+    if ((m.a[i].crec != nullptr) != rec) return hipErrorNotSupported;
+  return dispatch_ns(gas[0].g->ns, [&](auto nsc) -> hipError_t {
+    constexpr int NS = decltype(nsc)::value;
+    auto kern = k_gather_tiled_many<CT, D, NS, false>;
+    if constexpr (D == 2)
+      if (rec) kern = k_gather_tiled_many<CT, D, NS, true>;
+    if (lmax > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lmax);
+    hipLaunchKernelGGL(kern, dim3((unsigned)m.first[count], (unsigned)gas[0].n_trans), dim3(256), lmax, s, m);
+    return hipGetLastError();
+  });
+}
+
 }  // namespace
+
+hipError_t launch_interp_many(const GridOpArgs* gas, int count, const void* const* fw, void* const* c, hipStream_t s) {
+  if (count < 1 || count > kMaxMany) return hipErrorNotSupported;
+  const GridOpArgs& g0 = gas[0];
+  if (g0.prec != kF32 || g0.g->dim < 2 || use_mfma()) return hipErrorNotSupported;
+  for (int i = 0; i < count; ++i) {
+    const GridOpArgs& g = gas[i];
+    if (g.prec != kF32 || g.g->dim != g0.g->dim || g.g->ns != g0.g->ns || g.n_trans != g0.n_trans ||
+        g.ps->coord_f64 != g0.ps->coord_f64 || g.beta != g0.beta || g.c != g0.c || g.ps->M == 0 ||
+        g.g->tile_cells != g0.g->tile_cells)  // one LDS size for the whole launch (see launch_spread_many)
+      return hipErrorNotSupported;
+  }
+  const bool f64 = g0.ps->coord_f64;
+  if (g0.g->dim == 2)
+    return f64 ? interp_many_impl<double, 2>(gas, count, fw, c, s) : interp_many_impl<float, 2>(gas, count, fw, c, s);
+  return f64 ? interp_many_impl<double, 3>(gas, count, fw, c, s) : interp_many_impl<float, 3>(gas, count, fw, c, s);
+}
 
 hipError_t launch_interp(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
   if (ga.ps->M == 0) return hipSuccess;

@@ -675,6 +675,59 @@ int nufftx_execute(nufftx_plan p, void* c, void* f) {
   return nufftx_execute_sign(p, c, f, p->isign);
 }
 
+int nufftx_execute_many(int count, const nufftx_plan* plans, void* const* c, void* const* f) {
+  if (count < 1 || !plans || !c || !f) return fail(NUFFTX_ERR_INVALID_ARG, "execute_many: empty or null batch");
+  for (int i = 0; i < count; ++i) {
+    if (int r = check_plan(plans[i])) return r;
+    if (plans[i]->type != plans[0]->type || (plans[i]->type != 1 && plans[i]->type != 2))
+      return fail(NUFFTX_ERR_INVALID_ARG, "execute_many: all plans must be type 1, or all type 2");
+    if (!plans[i]->pts_set) return fail(NUFFTX_ERR_STATE, "setpts has not been called");
+    if ((!c[i] && plans[i]->M > 0) || !f[i]) return fail(NUFFTX_ERR_INVALID_ARG, "null data pointer");
+    for (int k = 0; k < i; ++k)
+      if (plans[k] == plans[i]) return fail(NUFFTX_ERR_INVALID_ARG, "execute_many: a plan appears twice");
+  }
+  bool shared = count > 1 && count <= kMaxManyPlans;
+  for (int i = 1; i < count && shared; ++i) shared = plans[i]->stream == plans[0]->stream;
+  if (!shared) {
+    for (int i = 0; i < count; ++i)
+      if (int r = nufftx_execute(plans[i], c[i], f[i])) return r;
+    return NUFFTX_OK;
+  }
+  const hipStream_t st = plans[0]->stream;
+  GridOpArgs gas[kMaxManyPlans];
+  void* grids[kMaxManyPlans];
+  for (int i = 0; i < count; ++i) {
+    gas[i] = gop(plans[i], plans[i]->ps, plans[i]->g);
+    grids[i] = plans[i]->grid;
+  }
+  if (plans[0]->type == 1) {
+    for (int i = 0; i < count; ++i) {
+      const nufftx_plan p = plans[i];
+      LAUNCH(launch_zero(p->grid, p->g.grid_cells() * p->n_trans * (int64_t)p->csize(), st));
+    }
+    hipError_t e = launch_spread_many(gas, count, c, grids, st);
+    if (e == hipErrorNotSupported) {
+      (void)hipGetLastError();
+      for (int i = 0; i < count; ++i) LAUNCH(launch_spread(gas[i], c[i], grids[i], st));
+    } else {
+      LAUNCH(e);
+    }
+    for (int i = 0; i < count; ++i)
+      if (int r = grid_to_modes(plans[i], f[i], plans[i]->isign)) return r;
+    return NUFFTX_OK;
+  }
+  for (int i = 0; i < count; ++i)
+    if (int r = modes_to_grid(plans[i], f[i], plans[i]->isign)) return r;
+  hipError_t e = launch_interp_many(gas, count, grids, c, st);
+  if (e == hipErrorNotSupported) {
+    (void)hipGetLastError();
+    for (int i = 0; i < count; ++i) LAUNCH(launch_interp(gas[i], grids[i], c[i], st));
+  } else {
+    LAUNCH(e);
+  }
+  return NUFFTX_OK;
+}
+
 int nufftx_vjp(nufftx_plan p, const void* src, const void* g, void* grad_src, void* gx, void* gy, void* gz) {
   if (int r = check_plan(p)) return r;
   if (!p->pts_set) return fail(NUFFTX_ERR_STATE, "setpts has not been called");

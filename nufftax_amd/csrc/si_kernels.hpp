@@ -393,9 +393,10 @@ __device__ __forceinline__ void bank_order(const SIArgs<float>& a, const int4& s
   __syncthreads();
 }
 
+// One block of the bank-spread kernel: subproblem sub of transform tr.
 template <typename CT, int D, int NS, bool REC, int C>
-__global__ void __launch_bounds__(1024) k_spread_banked(SIArgs<float> a, int maxsub, const cplx<float>* __restrict__ c,
-                                                        cplx<float>* __restrict__ fw) {
+__device__ __forceinline__ void spread_banked_block(const SIArgs<float>& a, int maxsub, int sub, int tr,
+                                                    const cplx<float>* __restrict__ c, cplx<float>* __restrict__ fw) {
   using Word = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Word* sre = reinterpret_cast<Word*>(smem_raw);
@@ -404,9 +405,7 @@ __global__ void __launch_bounds__(1024) k_spread_banked(SIArgs<float> a, int max
   int* sp = reinterpret_cast<int*>(sw + (size_t)D * maxsub);
   unsigned short* order = reinterpret_cast<unsigned short*>(sp + maxsub);
   __shared__ int ccnt[C], cfill[C];
-  const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
-  const int tr = blockIdx.y;
   const int4 sd = a.subprob[sub];
   const int n = sd.z;
   int org[3];
@@ -486,6 +485,39 @@ __global__ void __launch_bounds__(1024) k_spread_banked(SIArgs<float> a, int max
     atomic_add_global(&fT[gi].re, vr);
     atomic_add_global(&fT[gi].im, vi);
   }
+}
+
+template <typename CT, int D, int NS, bool REC, int C>
+__global__ void __launch_bounds__(1024) k_spread_banked(SIArgs<float> a, int maxsub, const cplx<float>* __restrict__ c,
+                                                        cplx<float>* __restrict__ fw) {
+  spread_banked_block<CT, D, NS, REC, C>(a, maxsub, (int)blockIdx.x, (int)blockIdx.y, c, fw);
+}
+
+// Several point sets (plans of one kernel size and dimension, any points and grids) in one
+// launch: plan p owns blocks [first[p], first[p+1]) of the x dimension.
+constexpr int kMaxMany = kMaxManyPlans;
+
+struct SpreadMany {
+  SIArgs<float> a[kMaxMany];
+  const cplx<float>* c[kMaxMany];
+  cplx<float>* fw[kMaxMany];
+  int first[kMaxMany + 1];
+  int count, maxsub;
+};
+static_assert(sizeof(SpreadMany) <= 4096, "kernel argument block too large");
+
+__device__ __forceinline__ int many_plan(const int* first, int count, int b) {
+  int p = 0;
+  for (int q = 1; q < count; ++q)
+    if (first[q] <= b) p = q;
+  return p;
+}
+
+template <typename CT, int D, int NS, bool REC, int C>
+__global__ void __launch_bounds__(1024) k_spread_banked_many(SpreadMany m) {
+  const int p = many_plan(m.first, m.count, (int)blockIdx.x);
+  spread_banked_block<CT, D, NS, REC, C>(m.a[p], m.maxsub, (int)blockIdx.x - m.first[p], (int)blockIdx.y, m.c[p],
+                                         m.fw[p]);
 }
 
 // ------------------------------------------------------------------ spread (global)
@@ -649,14 +681,12 @@ __device__ __forceinline__ void emit_point(const SIArgs<T>& a, const GatherOut<T
 
 // ------------------------------------------------------------------ interp / grad (tiled)
 // REC (2D FP32, values only): points come from the sorted point records.
-template <typename T, typename CT, int D, int NS, bool DERIV, bool REC = false>
-__global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>* __restrict__ fw,
-                                                      GatherOut<T, D, DERIV> o) {
+template <typename T, typename CT, int D, int NS, bool DERIV, bool REC>
+__device__ __forceinline__ void gather_tiled_block(const SIArgs<T>& a, const cplx<T>* __restrict__ fw,
+                                                   const GatherOut<T, D, DERIV>& o, int sub, int tr) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tile = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
-  const int tr = blockIdx.y;
   const int4 sd4 = a.subprob[sub];
   int org[3];
   tile_origin<D>(sd4, a.bs, a.nb, NS / 2, org);
@@ -698,6 +728,33 @@ __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>
     gather_point<T, D, NS, DERIV>(w, dw, fetch, s0, sdd);
     emit_point<T, D, DERIV>(a, o, tr, p, s0, sdd);
   }
+}
+
+template <typename T, typename CT, int D, int NS, bool DERIV, bool REC = false>
+__global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>* __restrict__ fw,
+                                                      GatherOut<T, D, DERIV> o) {
+  gather_tiled_block<T, CT, D, NS, DERIV, REC>(a, fw, o, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// Values-only FP32 interpolation of several point sets in one launch (see SpreadMany).
+struct GatherMany {
+  SIArgs<float> a[kMaxMany];
+  const cplx<float>* fw[kMaxMany];
+  cplx<float>* out[kMaxMany];
+  int first[kMaxMany + 1];
+  int count;
+};
+static_assert(sizeof(GatherMany) <= 4096, "kernel argument block too large");
+
+template <typename CT, int D, int NS, bool REC>
+__global__ void __launch_bounds__(256) k_gather_tiled_many(GatherMany m) {
+  const int p = many_plan(m.first, m.count, (int)blockIdx.x);
+  GatherOut<float, D, false> o;
+  o.val = m.out[p];
+  o.P = nullptr;
+  o.gcoord[0] = o.gcoord[1] = o.gcoord[2] = nullptr;
+  gather_tiled_block<float, CT, D, NS, false, REC>(m.a[p], m.fw[p], o, (int)blockIdx.x - m.first[p],
+                                                   (int)blockIdx.y);
 }
 
 template <typename T, typename CT, int D, int NS, bool DERIV>

@@ -112,7 +112,78 @@ hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t
   return hipErrorInvalidValue;
 }
 
+// Whether plan ga can join a many-plan bank-spread launch; lds gets its LDS bytes.
+template <int D>
+bool banked_ok(const GridOpArgs& ga, size_t& lds) {
+  size_t tl = 0;
+  if (!use_tiles(*ga.g, ga.prec, tl, true)) return false;
+  const PointSet& ps = *ga.ps;
+  lds = banked_lds<D>(ga.g->tile_cells, ps.maxsub);
+  if (ps.maxsub > 65535 || lds > (size_t)lds_device_max_bytes()) return false;
+  for (int d = 0; d < D; ++d)
+    if (ga.g->bs[d] > 127) return false;
+  return true;
+}
+
+template <typename CT, int D>
+hipError_t spread_many_impl(const GridOpArgs* gas, int count, const void* const* c, void* const* fw, hipStream_t s) {
+  size_t lmax = 0;
+  for (int i = 0; i < count; ++i) {
+    size_t l = 0;
+    if (!banked_ok<D>(gas[i], l)) return hipErrorNotSupported;
+    lmax = std::max(lmax, l);
+  }
+  SpreadMany m;
+  m.count = count;
+  m.maxsub = gas[0].ps->maxsub;
+  m.first[0] = 0;
+  for (int i = 0; i < count; ++i) {
+    hipError_t e = strength_absmax(gas[i], c[i], s);
+    if (e != hipSuccess) return e;
+    m.a[i] = make_si_args<float>(gas[i]);
+    m.a[i].cmax = gas[i].ps->cmax;
+    m.c[i] = static_cast<const cplx<float>*>(c[i]);
+    m.fw[i] = static_cast<cplx<float>*>(fw[i]);
+    m.first[i + 1] = m.first[i] + std::max(0, gas[i].ps->max_subprob);
+  }
+  for (int i = count; i < kMaxMany; ++i) m.first[i + 1] = m.first[count];
+  if (m.first[count] == 0) return hipSuccess;
+  const bool rec = m.a[0].crec != nullptr;
+  for (int i = 1; i < count; ++i)
+    if ((m.a[i].crec != nullptr) != rec) return hipErrorNotSupported;
+  const unsigned nt = 2 * lmax > (size_t)lds_device_max_bytes() ? 1024 : 256;
+  return dispatch_ns(gas[0].g->ns, [&](auto nsc) -> hipError_t {
+    constexpr int NS = decltype(nsc)::value;
+    auto kb = k_spread_banked_many<CT, D, NS, false, kBankClasses>;
+    if constexpr (D == 2)
+      if (rec) kb = k_spread_banked_many<CT, D, NS, true, kBankClasses>;
+    if (lmax > 65536) (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lmax);
+    hipLaunchKernelGGL(kb, dim3((unsigned)m.first[count], (unsigned)gas[0].n_trans), dim3(nt), lmax, s, m);
+    return hipGetLastError();
+  });
+}
+
 }  // namespace
+
+hipError_t launch_spread_many(const GridOpArgs* gas, int count, const void* const* c, void* const* fw, hipStream_t s) {
+  if (count < 1 || count > kMaxMany) return hipErrorNotSupported;
+  const GridOpArgs& g0 = gas[0];
+  if (g0.prec != kF32 || g0.g->dim < 2 || use_mfma()) return hipErrorNotSupported;
+  for (int i = 1; i < count; ++i) {
+    const GridOpArgs& g = gas[i];
+    if (g.prec != kF32 || g.g->dim != g0.g->dim || g.g->ns != g0.g->ns || g.n_trans != g0.n_trans ||
+        g.ps->coord_f64 != g0.ps->coord_f64 || g.ps->maxsub != g0.ps->maxsub || g.beta != g0.beta || g.c != g0.c)
+      return hipErrorNotSupported;
+  }
+  // a shared launch sizes its LDS for the largest tile; plans with other tile shapes would
+  // run their blocks at that (lower) occupancy, so they launch separately
+  for (int i = 0; i < count; ++i)
+    if (gas[i].ps->M == 0 || gas[i].g->tile_cells != g0.g->tile_cells) return hipErrorNotSupported;
+  const bool f64 = g0.ps->coord_f64;
+  if (g0.g->dim == 2)
+    return f64 ? spread_many_impl<double, 2>(gas, count, c, fw, s) : spread_many_impl<float, 2>(gas, count, c, fw, s);
+  return f64 ? spread_many_impl<double, 3>(gas, count, c, fw, s) : spread_many_impl<float, 3>(gas, count, c, fw, s);
+}
 
 hipError_t launch_spread(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
   if (ga.ps->M == 0) return hipSuccess;

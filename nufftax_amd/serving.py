@@ -7,18 +7,23 @@ batching of ``nufftax/core/spread.py:100-173`` and the batcher fast path of
 * requests are type-1 or type-2 transforms (1D/2D/3D, complex64/complex128) whose
   inputs already live on the GPU; ``submit`` returns a ``concurrent.futures.Future``;
 * a dispatcher thread admits requests against an HBM budget (points, strengths,
-  fine grid and output of every request in flight), groups requests with the
-  same plan signature that share their point set into one n_trans launch, and
-  issues each launch on one of ``n_streams`` HIP streams (round robin), each with
-  its own cached plans, so that small requests overlap on the GPU;
+  fine grid and output of every request in flight) and batches what is queued:
+  requests with the same plan signature that share their point set become one
+  n_trans unit; when at least ``batch_min_queue`` requests are queued, up to
+  ``max_units`` units of one kind, dimension, tolerance, sign and precision with
+  distinct grid sizes (any points, M up to ``batch_max_points``) run through
+  ``nufftx_execute_many``, which gives them one shared spreading (type 1) or
+  interpolation (type 2) launch; each batch is issued on one of ``n_streams`` HIP
+  streams (round robin),
+  each with its own pool of plans, so that batches overlap on the GPU;
 * a completion thread waits on each launch's HIP event in issue order per
   stream, resolves the futures and releases the budget;
 * every request records its arrival, start and completion time (host clock),
   from which ``latency_stats`` reports p50 / p99; ``future.request`` is the
   ``ServeRequest``.
 
-Results are those of the same plan called directly: the launch path is
-``nufftx_setpts`` + ``nufftx_execute`` on the request's stream.
+Results are those of the same plans called one by one: the launch path is
+``nufftx_setpts`` per unit + ``nufftx_execute`` (one unit) or ``nufftx_execute_many``.
 """
 
 from __future__ import annotations
@@ -56,14 +61,22 @@ class ServeRequest:
     t_start: float = 0.0         # taken by the dispatcher
     t_issued: float = 0.0        # all of its launches queued on the stream
     t_done: float = 0.0
+    _sig: tuple = None           # signature(), computed once
 
     @property
     def dim(self):
         return len(self.coords)
 
     def signature(self):
-        prec = F64 if (self.data.dtype == torch.complex128 or self.eps < 1e-7) else F32
-        return (self.kind, self.dim, tuple(self.n_modes), float(self.eps), int(self.isign), prec)
+        if self._sig is None:
+            prec = F64 if (self.data.dtype == torch.complex128 or self.eps < 1e-7) else F32
+            self._sig = (self.kind, self.dim, tuple(self.n_modes), float(self.eps), int(self.isign), prec)
+        return self._sig
+
+    def batch_key(self):
+        """Requests with equal keys may share a spreading / interpolation launch."""
+        kind, dim, _, eps, isign, prec = self.signature()
+        return (kind, dim, eps, isign, prec)
 
 
 def request_bytes(kind, dim, M, n_modes, eps, complex_bytes=8, n_trans=1):
@@ -78,19 +91,29 @@ def request_bytes(kind, dim, M, n_modes, eps, complex_bytes=8, n_trans=1):
 class NufftServer:
     """Asynchronous NUFFT executor over ``n_streams`` HIP streams with HBM admission control."""
 
-    def __init__(self, device="cuda:0", n_streams: int = 4, hbm_budget_bytes: int = 32 << 30, max_group: int = 64):
+    def __init__(self, device="cuda:0", n_streams: int = 4, hbm_budget_bytes: int = 32 << 30, max_group: int = 64,
+                 max_units: int = 8, batch_max_points: int = 200_000):
         if not torch.cuda.is_available():
             raise _lib.NufftxError("NufftServer needs an AMD GPU")
         self.device = torch.device(device)
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(n_streams)]  # default priority
         self.budget = int(hbm_budget_bytes)
         self.max_group = int(max_group)
+        self.max_units = max(1, min(int(max_units), 8))  # point sets per shared launch (library limit 8)
+        # only requests this small share launches with other point sets: their kernels are
+        # launch-latency bound; larger ones fill the GPU alone and keep the streams' overlap
+        self.batch_max_points = int(batch_max_points)
+        # queued requests looked at when forming one group: the scan runs on the dispatcher
+        # thread, whose issue rate bounds the throughput of small requests
+        self.scan_depth = 16
+        self.batch_min_queue = 4  # queued requests before point sets share launches
         self._in_use = 0
         self._cv = threading.Condition()
         self._q: "queue.Queue[ServeRequest | None]" = queue.Queue()
         self._done_q: "queue.Queue" = queue.Queue()
         self._next_stream = 0
-        self._plans = {}  # (signature, n_trans, stream) -> Plan, kept for the server's lifetime
+        self._plans = {}  # (signature, n_trans, stream) -> [Plan, ...], kept for the server's lifetime
+        self.plans_built = 0
         self._closed = False
         self._dispatcher = threading.Thread(target=self._dispatch_loop, name="nufftx-dispatch", daemon=True)
         self._completer = threading.Thread(target=self._complete_loop, name="nufftx-complete", daemon=True)
@@ -160,36 +183,67 @@ class NufftServer:
             self._in_use -= nbytes
             self._cv.notify_all()
 
-    def _take_group(self, first: ServeRequest):
-        """first plus queued requests with the same signature and the same point tensors."""
-        group = [first]
+    def _take_group(self, pending: list):
+        """pending[0] plus the requests after it (at most ``scan_depth`` looked at) that can run
+        with it: units of requests that share their point tensors and plan signature (one
+        n_trans launch each), and up to ``max_units`` such units of one kind, dimension,
+        tolerance, sign and precision (one shared spreading / interpolation launch).  Returns
+        (units, the requests left pending, in arrival order)."""
+        first = pending[0]
+        units = [[first]]
         held = []
         total = first.nbytes
-        while len(group) < self.max_group:
-            try:
-                r = self._q.get_nowait()
-            except queue.Empty:
+        n = 1
+        # other point sets join only under load: with a short queue the shared launch only
+        # delays the first request (measured: higher p50 at 1000 requests/s, no throughput gain)
+        cross = len(pending) >= self.batch_min_queue and self._batchable(first)
+        for idx in range(1, len(pending)):
+            r = pending[idx]
+            if n >= self.max_group or idx > self.scan_depth:
+                held.extend(pending[idx:])
                 break
-            if r is None:
+            if r.stream_index != first.stream_index or total + r.nbytes > self.budget or \
+                    r.batch_key() != first.batch_key():
                 held.append(r)
-                break
-            if r.signature() == first.signature() and all(a is b for a, b in zip(r.coords, first.coords)) and \
-                    r.stream_index == first.stream_index and \
-                    r.data.shape == first.data.shape and total + r.nbytes <= self.budget:
-                group.append(r)
-                total += r.nbytes
+                continue
+            unit = next((u for u in units if r.signature() == u[0].signature() and r.data.shape == u[0].data.shape
+                         and all(a is b for a, b in zip(r.coords, u[0].coords))), None)
+            if unit is not None:
+                unit.append(r)
+            elif cross and len(units) < self.max_units and self._batchable(r) and \
+                    all(r.signature() != u[0].signature() for u in units):
+                # one unit per plan signature, so that every unit of a batch runs on the stream's
+                # pooled plan of its signature (no plan is built for a second unit of one shape)
+                units.append([r])
             else:
                 held.append(r)
-        return group, held
+                continue
+            total += r.nbytes
+            n += 1
+        return units, held
+
+    def _batchable(self, r: ServeRequest) -> bool:
+        return r.coords[0].shape[-1] <= self.batch_max_points
 
     def _dispatch_loop(self):
-        pending: list = []
+        pending: list = []  # queued requests in arrival order
+        stopping = False
         while True:
-            r = pending.pop(0) if pending else self._q.get()
-            if r is None:
+            if not pending and not stopping:
+                pending.append(self._q.get())
+            while True:  # everything queued so far joins the pending list
+                try:
+                    pending.append(self._q.get_nowait())
+                except queue.Empty:
+                    break
+            if None in pending:  # close(): finish what was submitted before it
+                stopping = True
+                pending = [p for p in pending if p is not None]
+            if not pending:
                 break
-            group, held = self._take_group(r)
-            pending.extend(held)
+            units, pending = self._take_group(pending)
+            r = units[0][0]
+            group = [g for u in units for g in u]
             nbytes = sum(g.nbytes for g in group)
             self._admit(nbytes)
             if r.stream_index is not None:
@@ -202,7 +256,7 @@ class NufftServer:
                 with torch.cuda.stream(stream):
                     for g in group:
                         stream.wait_event(g.ready)
-                    outs = self._launch(group)
+                    outs = self._launch(units)
                     ev = torch.cuda.Event()
                     ev.record(stream)
             except Exception as e:  # noqa: BLE001 - reported through the futures
@@ -215,40 +269,66 @@ class NufftServer:
                 g.t_start, g.t_issued = t0, t1
             self._done_q.put((ev, group, outs, nbytes))
 
-    def _launch(self, group):
-        first = group[0]
-        kind, dim, n_modes, eps, isign, prec = first.signature()
-        cdt = torch.complex128 if prec == F64 else torch.complex64
-        rdt = torch.float64 if prec == F64 else torch.float32
-        coords = [x if (x.dtype == rdt and x.is_contiguous()) else x.to(rdt).contiguous() for x in first.coords]
-        M = coords[0].shape[0]
-        if kind == 1:
-            datas = [g.data.reshape(-1, M) for g in group]
+    def _plan(self, sig, n_trans, k):
+        """The k-th pooled plan of this signature, n_trans and the current stream."""
+        kind, dim, n_modes, eps, isign, prec = sig
+        key = (sig, n_trans, torch.cuda.current_stream(self.device).cuda_stream)
+        pool = self._plans.setdefault(key, [])
+        while len(pool) <= k:
+            pool.append(Plan.transform(kind, dim, n_modes, n_trans, eps, isign, 2.0, prec, 0, self.device))
+            self.plans_built += 1
+        return pool[k]
+
+    def _launch(self, units):
+        """Issue the units of one batch on the current stream; returns the outputs in request order."""
+        kind = units[0][0].kind
+        plans, srcs, dsts, parts = [], [], [], []
+        used = {}
+        for unit in units:
+            first = unit[0]
+            sig = first.signature()
+            _, dim, n_modes, _, _, prec = sig
+            cdt = torch.complex128 if prec == F64 else torch.complex64
+            rdt = torch.float64 if prec == F64 else torch.float32
+            coords = [x if (x.dtype == rdt and x.is_contiguous()) else x.to(rdt).contiguous() for x in first.coords]
+            M = coords[0].shape[0]
+            if kind == 1:
+                datas = [g.data.reshape(-1, M) for g in unit]
+            else:
+                datas = [g.data.reshape((-1,) + tuple(reversed(n_modes))) for g in unit]
+            sizes = [d.shape[0] for d in datas]
+            src = torch.cat(datas, 0).to(cdt).contiguous() if len(datas) > 1 else datas[0].to(cdt).contiguous()
+            n_trans = src.shape[0]
+            k = used.get((sig, n_trans), 0)
+            used[(sig, n_trans)] = k + 1
+            plan = self._plan(sig, n_trans, k)
+            plan.setpts(coords)
+            if kind == 1:
+                out = torch.empty((n_trans,) + tuple(reversed(n_modes)), dtype=cdt, device=self.device)
+            else:
+                out = torch.empty((n_trans, M), dtype=cdt, device=self.device)
+            plans.append(plan)
+            srcs.append(src)
+            dsts.append(out)
+            parts.append((unit, sizes, n_modes, M, dim))
+        if len(plans) == 1:
+            if kind == 1:
+                plans[0].execute(srcs[0], dsts[0])
+            else:
+                plans[0].execute(dsts[0], srcs[0])
+        elif kind == 1:
+            Plan.execute_many(plans, srcs, dsts)
         else:
-            datas = [g.data.reshape((-1,) + tuple(reversed(n_modes))) for g in group]
-        sizes = [d.shape[0] for d in datas]
-        src = torch.cat(datas, 0).to(cdt).contiguous() if len(datas) > 1 else datas[0].to(cdt).contiguous()
-        n_trans = src.shape[0]
-        key = (first.signature(), n_trans, torch.cuda.current_stream(self.device).cuda_stream)
-        plan = self._plans.get(key)
-        if plan is None:
-            plan = Plan.transform(kind, dim, n_modes, n_trans, eps, isign, 2.0, prec, 0, self.device)
-            self._plans[key] = plan
-        plan.setpts(coords)
-        if kind == 1:
-            out = torch.empty((n_trans,) + tuple(reversed(n_modes)), dtype=cdt, device=self.device)
-            plan.execute(src, out)
-        else:
-            out = torch.empty((n_trans, M), dtype=cdt, device=self.device)
-            plan.execute(out, src)
+            Plan.execute_many(plans, dsts, srcs)
         outs = []
-        off = 0
-        for g, n in zip(group, sizes):
-            o = out[off:off + n]
-            off += n
-            lead = g.data.shape[:-1] if kind == 1 else g.data.shape[: g.data.ndim - dim]
-            tail = tuple(reversed(n_modes)) if kind == 1 else (M,)
-            outs.append(o.reshape(tuple(lead) + tail).to(g.data.dtype))
+        for out, (unit, sizes, n_modes, M, dim) in zip(dsts, parts):
+            off = 0
+            for g, n in zip(unit, sizes):
+                o = out[off:off + n]
+                off += n
+                lead = g.data.shape[:-1] if kind == 1 else g.data.shape[: g.data.ndim - dim]
+                tail = tuple(reversed(n_modes)) if kind == 1 else (M,)
+                outs.append(o.reshape(tuple(lead) + tail).to(g.data.dtype))
         return outs
 
     def _complete_loop(self):
@@ -322,10 +402,11 @@ def _direct_call(kind, coords, data, nm, eps=1e-6):
     return fn(*coords, data, eps=eps)
 
 
-def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4):
+def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4, max_units: int = 8):
     """Replay reqs with Poisson arrivals at qps (qps <= 0: every request arrives at once, which
-    measures the saturated throughput).  mode "server": NufftServer; mode "per-call": the same
-    transforms called one at a time (each synchronised) in arrival order."""
+    measures the saturated throughput).  mode "server": NufftServer (max_units point sets per
+    shared launch; 1 turns cross-point batching off); mode "per-call": the same transforms called
+    one at a time (each synchronised) in arrival order."""
     rng = np.random.default_rng(seed + 1)
     if qps > 0:
         arrivals = np.cumsum(rng.exponential(1.0 / qps, len(reqs)))
@@ -335,8 +416,9 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
     torch.cuda.synchronize(dev)
     records = []
     if mode == "server":
-        with NufftServer(dev, n_streams=n_streams) as srv:
-            # build every stream's plans for the distinct shapes before the timed stream
+        with NufftServer(dev, n_streams=n_streams, max_units=max_units) as srv:
+            # build every stream's plans for the distinct shapes, then replay the stream once
+            # untimed (all at once, the largest batches) so that the plan pools are filled
             firsts = {}
             for r in reqs:
                 firsts.setdefault((r[0], len(r[1]), r[3]), r)
@@ -345,7 +427,11 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
                         for k, c, d, nm in firsts.values()]
                 for f in warm:
                     f.result()
+            warm = [srv.submit(k, c, d, n_modes=nm if k == 1 else None, eps=1e-6) for k, c, d, nm in reqs]
+            for f in warm:
+                f.result()
             torch.cuda.synchronize(dev)
+            built0 = srv.plans_built
             t0 = time.perf_counter()
             futs = []
             for (kind, coords, data, nm), ta in zip(reqs, arrivals):
@@ -358,6 +444,7 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
             for f in futs:
                 f.result()
             records = [f.request for f in futs]
+            built = srv.plans_built - built0
     else:
         t0 = time.perf_counter()
         for (kind, coords, data, nm), ta in zip(reqs, arrivals):
@@ -372,7 +459,10 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
             torch.cuda.synchronize(dev)
             r.t_done = time.perf_counter()
             records.append(r)
-    return latency_stats(records)
+    st = latency_stats(records)
+    if mode == "server":
+        st["plans_built_in_replay"] = built
+    return st
 
 
 def service_times(reqs) -> dict:
@@ -405,6 +495,8 @@ def main(argv=None):
                     help="one stream replay per rate; 0 = all requests at once (saturated throughput)")
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--compare-unbatched", action="store_true",
+                    help="also replay through a server that never shares a launch across point sets")
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
     reqs = mixed_requests(args.requests, args.seed, dev)
@@ -423,6 +515,8 @@ def main(argv=None):
         res = {"qps": qps, "streams": args.streams,
                "server": run_stream(reqs, qps, args.seed, "server", args.streams),
                "per_call": run_stream(reqs, qps, args.seed, "per-call")}
+        if args.compare_unbatched:
+            res["server_unbatched"] = run_stream(reqs, qps, args.seed, "server", args.streams, max_units=1)
         print(json.dumps(res), flush=True)
 
 

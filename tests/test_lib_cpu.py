@@ -201,3 +201,49 @@ def test_serving_host_logic():
     if not torch.cuda.is_available():
         with pytest.raises(Exception):
             S.NufftServer()
+
+
+def test_serving_group_formation():
+    """The dispatcher's grouping on host-side requests: same points and signature form one
+    unit, other small point sets of the same kind/dim/eps/sign join as further units (up to
+    max_units), everything else stays pending in arrival order."""
+    import torch
+
+    from nufftax_amd import serving as S
+
+    srv = object.__new__(S.NufftServer)
+    srv.max_group, srv.max_units, srv.batch_max_points, srv.scan_depth, srv.batch_min_queue = 64, 3, 1000, 256, 4
+    srv.budget = 1 << 30
+
+    def req(kind, pts, nm=(16, 16), big=False):
+        M = pts[0].shape[0]
+        data = torch.zeros(M, dtype=torch.complex64) if kind == 1 else torch.zeros(nm, dtype=torch.complex64)
+        r = S.ServeRequest(kind, pts, data, nm, 1e-6, 1 if kind == 1 else -1)
+        r.nbytes = 100
+        return r
+
+    p1 = (torch.zeros(100), torch.zeros(100))
+    p2 = (torch.zeros(200), torch.zeros(200))
+    p3 = (torch.zeros(300), torch.zeros(300))
+    p4 = (torch.zeros(400), torch.zeros(400))
+    big = (torch.zeros(5000), torch.zeros(5000))
+    a, b = req(1, p1), req(1, p1)              # one unit (shared points)
+    t2 = req(2, p2)                            # other kind: pending
+    c = req(1, p2, nm=(32, 32))                # second unit (other grid size)
+    d = req(1, big)                            # too many points to share a launch: pending
+    e = req(1, p3, nm=(8, 8))                  # third unit
+    f = req(1, p4, nm=(4, 4))                  # over max_units: pending
+    g = req(1, p4)                             # a second unit of a's shape: pending
+    units, held = srv._take_group([a, t2, b, g, c, d, e, f])
+    assert [[id(r) for r in u] for u in units] == [[id(a), id(b)], [id(c)], [id(e)]]
+    assert [id(r) for r in held] == [id(t2), id(g), id(d), id(f)]
+    # a large first request keeps its own points only
+    units, held = srv._take_group([d, a, b])
+    assert [[id(r) for r in u] for u in units] == [[id(d)]] and len(held) == 2
+    # a short queue: only the same points group
+    units, held = srv._take_group([a, c, b])
+    assert [[id(r) for r in u] for u in units] == [[id(a), id(b)]] and [id(r) for r in held] == [id(c)]
+    # the budget caps the group
+    srv.budget = 250
+    units, held = srv._take_group([a, b, c, e])
+    assert sum(len(u) for u in units) == 2 and [id(r) for r in held] == [id(c), id(e)]
