@@ -9,8 +9,8 @@ batching of ``nufftax/core/spread.py:100-173`` and the batcher fast path of
 * a dispatcher thread admits requests against an HBM budget (points, strengths,
   fine grid and output of every request in flight) and batches what is queued:
   requests with the same plan signature that share their point set become one
-  n_trans unit; when at least ``batch_min_queue`` requests are queued, up to
-  ``max_units`` units of one kind, dimension, tolerance, sign and precision with
+  n_trans unit; with ``max_units`` > 1 and at least ``batch_min_queue`` requests
+  queued, up to ``max_units`` units of one kind, dimension, tolerance, sign and precision with
   distinct grid sizes (any points, M up to ``batch_max_points``) run through
   ``nufftx_execute_many``, which gives them one shared spreading (type 1) or
   interpolation (type 2) launch; each batch is issued on one of ``n_streams`` HIP
@@ -92,14 +92,18 @@ class NufftServer:
     """Asynchronous NUFFT executor over ``n_streams`` HIP streams with HBM admission control."""
 
     def __init__(self, device="cuda:0", n_streams: int = 4, hbm_budget_bytes: int = 32 << 30, max_group: int = 64,
-                 max_units: int = 8, batch_max_points: int = 200_000):
+                 max_units: int = 1, batch_max_points: int = 200_000):
         if not torch.cuda.is_available():
             raise _lib.NufftxError("NufftServer needs an AMD GPU")
         self.device = torch.device(device)
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(n_streams)]  # default priority
         self.budget = int(hbm_budget_bytes)
         self.max_group = int(max_group)
-        self.max_units = max(1, min(int(max_units), 8))  # point sets per shared launch (library limit 8)
+        # point sets per shared launch (library limit 8).  Default 1: on the mixed stream the
+        # server is bound by its dispatcher thread's issue rate, and shared launches (fewer
+        # kernels, one stream per batch) measured 5% below separate ones at saturation
+        self.max_units = max(1, min(int(max_units), 8))
+        self.shared_launches = 0
         # only requests this small share launches with other point sets: their kernels are
         # launch-latency bound; larger ones fill the GPU alone and keep the streams' overlap
         self.batch_max_points = int(batch_max_points)
@@ -317,8 +321,10 @@ class NufftServer:
             else:
                 plans[0].execute(dsts[0], srcs[0])
         elif kind == 1:
+            self.shared_launches += 1
             Plan.execute_many(plans, srcs, dsts)
         else:
+            self.shared_launches += 1
             Plan.execute_many(plans, dsts, srcs)
         outs = []
         for out, (unit, sizes, n_modes, M, dim) in zip(dsts, parts):
@@ -402,7 +408,7 @@ def _direct_call(kind, coords, data, nm, eps=1e-6):
     return fn(*coords, data, eps=eps)
 
 
-def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4, max_units: int = 8):
+def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4, max_units: int = 1):
     """Replay reqs with Poisson arrivals at qps (qps <= 0: every request arrives at once, which
     measures the saturated throughput).  mode "server": NufftServer (max_units point sets per
     shared launch; 1 turns cross-point batching off); mode "per-call": the same transforms called
@@ -495,8 +501,10 @@ def main(argv=None):
                     help="one stream replay per rate; 0 = all requests at once (saturated throughput)")
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--compare-unbatched", action="store_true",
-                    help="also replay through a server that never shares a launch across point sets")
+    ap.add_argument("--max-units", type=int, default=1,
+                    help="point sets per shared spreading / interpolation launch in the server (1..8)")
+    ap.add_argument("--compare-batched", type=int, default=0, metavar="UNITS",
+                    help="also replay through a server with this many point sets per shared launch")
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
     reqs = mixed_requests(args.requests, args.seed, dev)
@@ -513,10 +521,11 @@ def main(argv=None):
                       "requests": args.requests, "service": svc}), flush=True)
     for qps in args.qps:
         res = {"qps": qps, "streams": args.streams,
-               "server": run_stream(reqs, qps, args.seed, "server", args.streams),
+               "server": run_stream(reqs, qps, args.seed, "server", args.streams, args.max_units),
                "per_call": run_stream(reqs, qps, args.seed, "per-call")}
-        if args.compare_unbatched:
-            res["server_unbatched"] = run_stream(reqs, qps, args.seed, "server", args.streams, max_units=1)
+        if args.compare_batched:
+            res[f"server_units{args.compare_batched}"] = run_stream(reqs, qps, args.seed, "server", args.streams,
+                                                                   args.compare_batched)
         print(json.dumps(res), flush=True)
 
 

@@ -56,6 +56,36 @@ def test_shared_points_are_grouped_and_equal():
         assert _rel(o, N.nufft2d1(x, y, c, (64, 48), eps=1e-6)) < 1e-6
 
 
+def test_shared_launches_across_point_sets_match_direct_calls():
+    """Uneven small requests (different points, M and grid sizes) through shared spreading /
+    interpolation launches equal the same requests called one by one."""
+    from nufftax_amd import serving as S
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    reqs = []
+    for i in range(24):
+        kind = 1 + (i % 2)
+        nm = [(32, 32), (64, 48), (96, 64), (128, 128)][(i // 2) % 4]
+        M = 3000 + 997 * i
+        pts = tuple((torch.rand(M, device=dev, generator=g) * 2 - 1) * np.pi for _ in range(2))
+        if kind == 1:
+            data = torch.randn(M, dtype=torch.complex64, device=dev, generator=g)
+        else:
+            data = torch.randn(tuple(reversed(nm)), dtype=torch.complex64, device=dev, generator=g)
+        reqs.append((kind, pts, data, nm))
+    with S.NufftServer(dev, n_streams=2, max_units=8) as srv:
+        srv.batch_min_queue = 1
+        futs = [srv.submit(k, c, d, n_modes=nm if k == 1 else None, eps=1e-6) for k, c, d, nm in reqs]
+        outs = [f.result(timeout=120) for f in futs]
+        shared = srv.shared_launches
+    for (k, c, d, nm), o in zip(reqs, outs):
+        ref = S._direct_call(k, c, d, nm)
+        assert o.shape == ref.shape and _rel(o, ref) < 1e-6
+    assert shared >= 1
+
+
 def test_admission_budget():
     from nufftax_amd import serving as S
 
