@@ -10,6 +10,10 @@ nufft2d2 (bin-sort + deconvolve/pad + FFT + interp) on the same points, inputs
 resident in HBM.  value = 2 * M / step time (NU points per second over both
 transforms).  The roofline object is for the dominant kernel (spread or
 interp), timed with HIP events on the plan stream.
+
+Secondary figures on the same JSON line (not part of value): config_3d (3D
+clustered type 1 / type 2, complex64, eps 1e-6 and 1e-4) and config_extra (the
+2D gradient round trip, 3D complex128 at eps 1e-12, 3D type 3), ms per call.
 """
 
 from __future__ import annotations
@@ -44,6 +48,8 @@ def _parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D secondary figures")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the gradient round trip, 3D complex128 and 3D type-3 figures")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000, help="points in the CPU-baseline sample")
     ap.add_argument("--cpu-runs", type=int, default=7, help="timed CPU-baseline runs (median reported)")
     ap.add_argument("--profile-only", action="store_true", help="run a few steps, print nothing but a status line")
@@ -100,6 +106,61 @@ def cpu_baseline(x, y, c, f, nm, eps, sample, runs):
         "sample": f"nufft2d1 + nufft2d2 on the first {sample} radial points (512x512 modes, complex64), "
                   f"median of {len(times)} runs, {t:.2f} s each",
     }
+
+
+def bench_extra(torch, dev):
+    """The remaining measured workloads, through the public transforms (plans cached, points
+    sorted in every call), ms per call (mean of 5 after one untimed call):
+    * 2D round trip with gradients: loss = sum |nufft2d1(x, y, nufft2d2(x, y, f)) - g|^2 on 1M
+      uniform points, 256^2 modes, complex64, eps 1e-6; forward + backward w.r.t. f, x, y;
+    * 3D complex128, eps 1e-12: 500k clustered points, 64^3 modes, type 1 and type 2;
+    * 3D type 3: 200k sources, 200k targets in [-64, 64]^3, complex128, eps 1e-9."""
+    import nufftax_amd as N
+    from nufftax_amd import workloads
+
+    def timeit(fn, n=5):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    out = {}
+    (x, y), _ = workloads.uniform_nd(seed=0, M=1_000_000, dim=2)
+    rng = np.random.default_rng(2)
+    f0 = rng.standard_normal((256, 256)) + 1j * rng.standard_normal((256, 256))
+    g = torch.from_numpy((rng.standard_normal((256, 256)) + 1j * rng.standard_normal((256, 256))).astype(np.complex64)).to(dev)
+    xs = torch.from_numpy(x.astype(np.float32)).to(dev).requires_grad_(True)
+    ys = torch.from_numpy(y.astype(np.float32)).to(dev).requires_grad_(True)
+    f = torch.from_numpy(f0.astype(np.complex64)).to(dev).requires_grad_(True)
+
+    def roundtrip():
+        for t in (xs, ys, f):
+            t.grad = None
+        o = N.nufft2d1(xs, ys, N.nufft2d2(xs, ys, f, eps=1e-6), (256, 256), eps=1e-6)
+        torch.sum(torch.abs(o - g) ** 2).backward()
+
+    out["roundtrip_2d_grad_1M_256sq_c64"] = {"forward_backward_ms": timeit(roundtrip)}
+
+    coords, c = workloads.gaussian_mixture_3d(seed=1, M=500_000)
+    xd = [torch.from_numpy(v).to(dev) for v in coords]
+    cd = torch.from_numpy(c).to(dev)
+    fd = torch.from_numpy((rng.standard_normal((64, 64, 64)) + 1j * rng.standard_normal((64, 64, 64)))).to(dev)
+    out["clusters_3d_500k_64cube_c128_eps1e-12"] = {
+        "nufft3d1_ms": timeit(lambda: N.nufft3d1(*xd, cd, (64, 64, 64), eps=1e-12)),
+        "nufft3d2_ms": timeit(lambda: N.nufft3d2(*xd, fd, eps=1e-12)),
+    }
+
+    src, tgt, c3 = workloads.type3_3d(seed=0)
+    nf = tuple(N.compute_type3_grid_size(src[d], tgt[d], eps=1e-9) for d in range(3))
+    sd = [torch.from_numpy(v).to(dev) for v in src]
+    td = [torch.from_numpy(v).to(dev) for v in tgt]
+    c3d = torch.from_numpy(c3).to(dev)
+    out["type3_3d_200k_c128_eps1e-9"] = {"fine_grid": list(nf),
+                                         "nufft3d3_ms": timeit(lambda: N.nufft3d3(*sd, c3d, *td, nf, eps=1e-9))}
+    return out
 
 
 def bench_3d(args, torch, dev):
@@ -309,6 +370,8 @@ def main():
     }
     if not args.no_3d:
         res["config_3d"] = bench_3d(args, torch, dev)
+    if not args.no_extra:
+        res["config_extra"] = bench_extra(torch, dev)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(x32, y32, c64, f64, nm, eps, args.cpu_sample, args.cpu_runs)
     print(json.dumps(res), flush=True)
