@@ -92,3 +92,38 @@ def test_execute_many_falls_back_for_mixed_kernel_widths(rng):
     for (x, d, nm), eps, o in zip(cases, (1e-6, 1e-3), outs):
         ref = N.nufft2d1(x[0], x[1], d, nm, eps=eps)
         assert R.rel_l2(o.cpu().numpy(), ref.cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_execute_many_n_trans_and_fp64_plans(rng, kind):
+    """two transforms per plan in the shared launch (FP32), and FP64 plans (plan-by-plan
+    path): each equals the plans executed alone"""
+    from nufftax_amd._plan import F32, F64, Plan
+
+    for prec, rdt, cdt, tol in ((F32, np.float32, np.complex64, 1e-6), (F64, np.float64, np.complex128, 1e-12)):
+        shapes, Ms = [(48, 40), (128, 96), (64, 64)], [7000, 30000, 1500]
+        plans, ins, alone = [], [], []
+        for nm, M in zip(shapes, Ms):
+            x = [torch.from_numpy(rng.uniform(-np.pi, np.pi, M).astype(rdt)).to(DEV) for _ in range(2)]
+            shp = (2, M) if kind == 1 else (2,) + tuple(reversed(nm))
+            d = torch.from_numpy((rng.standard_normal(shp) + 1j * rng.standard_normal(shp)).astype(cdt)).to(DEV)
+            p = Plan.transform(kind, 2, nm, 2, 1e-6 if prec == F32 else 1e-12, 1, 2.0, prec, 0, torch.device(DEV))
+            p.setpts(x)
+            oshape = (2,) + tuple(reversed(nm)) if kind == 1 else (2, M)
+            o = torch.empty(oshape, dtype=d.dtype, device=DEV)
+            if kind == 1:
+                p.execute(d, o)
+            else:
+                p.execute(o, d)
+            torch.cuda.synchronize()
+            alone.append(o.cpu().numpy())
+            plans.append(p)
+            ins.append(d)
+        outs = [torch.empty_like(torch.from_numpy(a)).to(DEV) for a in alone]
+        if kind == 1:
+            Plan.execute_many(plans, ins, outs)
+        else:
+            Plan.execute_many(plans, outs, ins)
+        torch.cuda.synchronize()
+        for a, o in zip(alone, outs):
+            assert R.rel_l2(o.cpu().numpy(), a) < tol
