@@ -22,8 +22,10 @@
 //      k_row_offsets), and walk their keys
 //      in index order, ranking equal keys inside each 64-key step (one comparison
 //      when the step's keys agree, else one ballot per key bit), and write
-//      perm[slot] = index.
+//      perm[slot] = index.  Above 1024 bins without point records, k_scatter16 keeps
+//      16-bit wave counters (two per LDS word) so that twice as many waves fit.
 #include <algorithm>
+#include <cstdlib>
 
 #include "internal.hpp"
 
@@ -371,6 +373,88 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
   }
 }
 
+// k_scatter16: k_scatter without records for more than 1024 bins, with 16-bit wave
+// counters (two per LDS word), so that twice as many waves fit a chunk (16384 bins: 4 waves
+// instead of 2).  A wave's counters hold offsets inside the chunk (below 65536: the host
+// keeps chunks that small); the first slot of (chunk, bin) comes from the row offsets.
+// Counting adds 1 << 16 (key & 1) to word key / 2 with a 32-bit LDS atomic; ranking lets the
+// lowest lane of each key add its count the same way and hands the old offset to the others.
+template <typename CT>
+__global__ void __launch_bounds__(1024) k_scatter16(const int32_t* __restrict__ keys, int64_t M, int SC, int nbins,
+                                                    int key_bits, ScanArgs sa, int32_t* __restrict__ perm) {
+  extern __shared__ uint32_t wc2[];  // [W][(nbins + 1) / 2]
+  {
+    const int n = min(*sa.nsub, sa.max_subprob);
+    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n; slot += gridDim.x * blockDim.x) {
+      int lo = 0, hi = nbins;  // invariant: sub_start[lo] <= slot < sub_start[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sa.sub_start[mid] <= slot)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const int k = slot - sa.sub_start[lo];
+      const int st = sa.bin_start[lo], cnt = sa.bin_start[lo + 1] - st;
+      sa.subprob[slot] = make_int4(lo, st + k * sa.maxsub, min(sa.maxsub, cnt - k * sa.maxsub), 0);
+    }
+  }
+  const int W = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nw2 = (nbins + 1) >> 1;
+  uint32_t* my = wc2 + (int64_t)wave * nw2;
+  for (int b = lane; b < nw2; b += 64) my[b] = 0u;
+  const int64_t lo = (int64_t)blockIdx.x * W * SC + (int64_t)wave * SC;
+  const int64_t hi = min(M, lo + (int64_t)SC);
+  for (int64_t base = lo; base < hi; base += kStep * kGroup) {
+    int k[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + u * kStep + lane;
+      k[u] = i < hi ? keys[i] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u)
+      if (k[u] >= 0) atomicAdd(&my[k[u] >> 1], 1u << (16 * (k[u] & 1)));
+  }
+  __syncthreads();
+  // offsets inside the chunk: earlier waves' counts of the bin (both halves at once: no half
+  // reaches 65536, so no carry crosses)
+  for (int b2 = threadIdx.x; b2 < nw2; b2 += blockDim.x) {
+    uint32_t run = 0u;
+    for (int w = 0; w < W; ++w) {
+      const uint32_t t = wc2[(int64_t)w * nw2 + b2];
+      wc2[(int64_t)w * nw2 + b2] = run;
+      run += t;
+    }
+  }
+  __syncthreads();
+  const int32_t* __restrict__ first = sa.H + (int64_t)blockIdx.x * nbins;  // k_row_offsets
+  for (int64_t base = lo; base < hi; base += kStep * kGroup) {
+    int k[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + u * kStep + lane;
+      k[u] = i < hi ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int64_t i = base + u * kStep + lane;
+      const bool valid = i < hi;
+      const int key = k[u];
+      const uint64_t m = match_key(key, valid, key_bits);
+      const int rank = popc_below(m);
+      const int sh = 16 * (key & 1);
+      uint32_t old = 0u;
+      if (valid && rank == 0) old = atomicAdd(&my[key >> 1], (uint32_t)__popcll(m) << sh);
+      const int leader = m ? __ffsll((long long)m) - 1 : lane;
+      const int off = __shfl((int)((old >> sh) & 0xFFFFu), leader, 64);
+      const int pos = first[key] + off + rank;
+      if (valid && pos >= 0 && (int64_t)pos < M) perm[pos] = (int32_t)i;
+    }
+  }
+}
+
 __global__ void k_zero_i32(int32_t* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -381,7 +465,8 @@ __global__ void k_empty_sort(int nbins, int32_t* bin_start, int32_t* nsub) {
 }
 
 template <typename CT, int D>
-void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int SC, const RecArgs& ra, hipStream_t s) {
+void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int SC, bool c16, const RecArgs& ra,
+                        hipStream_t s) {
   int key_bits = 0;
   while ((1 << key_bits) < nbins) ++key_bits;
   const int CH = W * SC;
@@ -409,6 +494,14 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
     hipLaunchKernelGGL(k_row_offsets, dim3((nbins + 255) / 256, sa.nseg), dim3(256), 0, s, sa);
   } else {
     hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(1024), 0, s, sa);
+  }
+  if (c16) {
+    const size_t l16 = (size_t)W * (size_t)((nbins + 1) / 2) * sizeof(uint32_t);
+    if (l16 > 65536)
+      (void)hipFuncSetAttribute((const void*)k_scatter16<CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l16);
+    hipLaunchKernelGGL((k_scatter16<CT>), dim3(ps.NB), dim3(64 * W), l16, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
+                       ps.perm);
+    return;
   }
   const size_t lds = (size_t)W * nbins * sizeof(int32_t);
   if (lds > 65536) {
@@ -444,14 +537,32 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   ps.maxsub = tune.maxsub;
   const int64_t M = ps.M;
   // waves per chunk (up to 16): their LDS counters (W * nbins ints) must fit the device's LDS
+  const int64_t lds_max = lds_device_max_bytes();
   int W = 16;
-  while (W > 1 && (int64_t)W * nbins * (int64_t)sizeof(int32_t) > (int64_t)lds_device_max_bytes()) W /= 2;
+  while (W > 1 && (int64_t)W * nbins * (int64_t)sizeof(int32_t) > lds_max) W /= 2;
   // points per wave: chunk / W, and enough that the histogram fits hist_cap
-  int64_t SC = std::max<int64_t>(tune.chunk / W, 64);
   const int64_t need = (M * (int64_t)nbins + tune.hist_cap - 1) / tune.hist_cap;  // min points per chunk
-  SC = std::max<int64_t>(SC, (need + W - 1) / W);
-  SC = (SC + 63) / 64 * 64;
-  if (SC > (1 << 24)) SC = 1 << 24;
+  auto wave_points = [&](int w) {
+    int64_t sc = std::max<int64_t>(tune.chunk / w, 64);
+    sc = std::max<int64_t>(sc, (need + w - 1) / w);
+    sc = (sc + 63) / 64 * 64;
+    return std::min<int64_t>(sc, 1 << 24);
+  };
+  int64_t SC = wave_points(W);
+  // 16-bit wave counters when 32-bit ones leave fewer than 16 waves per chunk (more than 1024
+  // bins, no records, chunks below 65536 points)
+  bool c16 = false;
+  const bool want_rec = (rec_kind == 1 || rec_kind == 2) && g.dim == 2;
+  if (W < 16 && nbins > 1024 && !want_rec && !std::getenv("NUFFTX_SORT_C32")) {
+    int W16 = 16;
+    while (W16 > 1 && (int64_t)W16 * ((nbins + 1) / 2) * (int64_t)sizeof(uint32_t) > lds_max) W16 /= 2;
+    const int64_t sc16 = wave_points(W16);
+    if (W16 > W && (int64_t)W16 * sc16 <= 65535) {
+      W = W16;
+      SC = sc16;
+      c16 = true;
+    }
+  }
   const int64_t NB = std::max<int64_t>(1, (M + W * SC - 1) / (W * SC));
   ps.NB = (int)NB;
   ps.CH = (int)(W * SC);
@@ -507,13 +618,13 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   ra.recs = recs ? ps.recs : nullptr;
   ra.crec = crec ? ps.crec : nullptr;
   if (ps.coord_f64) {
-    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, ra, s);
-    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, ra, s);
-    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, W, isc, ra, s);
+    if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, c16, ra, s);
+    if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, c16, ra, s);
+    if (g.dim == 3) launch_sort_passes<double, 3>(ka, ps, nbins, W, isc, c16, ra, s);
   } else {
-    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, W, isc, ra, s);
-    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, ra, s);
-    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, ra, s);
+    if (g.dim == 1) launch_sort_passes<float, 1>(ka, ps, nbins, W, isc, c16, ra, s);
+    if (g.dim == 2) launch_sort_passes<float, 2>(ka, ps, nbins, W, isc, c16, ra, s);
+    if (g.dim == 3) launch_sort_passes<float, 3>(ka, ps, nbins, W, isc, c16, ra, s);
   }
   e = hipGetLastError();
   if (e == hipSuccess && recs) ps.has_recs = true;

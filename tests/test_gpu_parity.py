@@ -43,7 +43,8 @@ def _need_gpu():
 
 
 @pytest.mark.parametrize("dim,n_modes,M", [(1, (1000,), 50000), (2, (100, 70), 30000), (2, (512, 512), 200000),
-                                           (3, (40, 32, 24), 40000), (3, (64, 64, 64), 200000)])
+                                           (2, (1024, 1024), 300000), (3, (40, 32, 24), 40000),
+                                           (3, (64, 64, 64), 200000), (3, (128, 128, 128), 300000)])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
 def test_bin_sort_bit_exact(rng, dim, n_modes, M, dist):
     from nufftax_amd._plan import F32, transform_plan
@@ -98,6 +99,15 @@ def test_bin_sort_one_bin_many_points(rng):
     y = np.full(M, -1.0)
     x[-3:] = 2.5
     _check_sort([x, y], (128, 128), kind=2)
+
+
+def test_bin_sort_clusters_8m_full_size():
+    """the 3D benchmark's point set (8M clustered points, 256^3 grid, 16384 bins: the 16-bit
+    wave-counter scatter), type-1 plan."""
+    from nufftax_amd import workloads
+
+    coords, _ = workloads.gaussian_mixture_3d(seed=0)
+    _check_sort(coords, (128, 128, 128), kind=1)
 
 
 def test_bin_sort_radial_4m_full_size():
