@@ -113,6 +113,7 @@ struct GridOpArgs {
   int prec;
   int64_t n_trans;
   double beta, c;
+  double tol = 0.0;  // the plan's requested accuracy (0: unknown), selects packed FP32 tiles
 };
 // fw[n_trans][grid] += spread(c[n_trans][M]); fw must be zeroed by the caller.
 hipError_t launch_spread(const GridOpArgs& a, const void* c, void* fw, hipStream_t s);

@@ -375,6 +375,7 @@ GridOpArgs gop(nufftx_plan p, const PointSet& ps, const Geometry& g) {
   a.n_trans = p->n_trans;
   a.beta = p->beta;
   a.c = p->kc;
+  a.tol = p->eps;
   return a;
 }
 

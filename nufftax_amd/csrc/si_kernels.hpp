@@ -318,8 +318,33 @@ __global__ void __launch_bounds__(256) k_spread_tiled(SIArgs<T> a, const cplx<T>
 constexpr int kBankClasses = 16;
 
 template <int D>
-inline size_t banked_lds(int64_t tile_cells, int maxsub) {
-  return (size_t)tile_cells * 16 + (size_t)maxsub * (D * sizeof(unsigned) + sizeof(int) + sizeof(unsigned short));
+inline size_t banked_lds(int64_t tile_cells, int maxsub, bool packed = false) {
+  return (size_t)tile_cells * (packed ? 8 : 16) +
+         (size_t)maxsub * (D * sizeof(unsigned) + sizeof(int) + sizeof(unsigned short));
+}
+
+// Packed FP32 tiles (plans with eps >= kPackedTol): one 64-bit word per cell holds the real
+// part in its high and the imaginary part in its low 32 bits, both signed fixed point, so a
+// support cell takes one ds_add_u64 instead of two.  Word = re 2^32 + im (mod 2^64) adds
+// exactly while every partial sum stays below 2^31 in magnitude: the scale is 2^(30 - e) with
+// 2^e above 2 n max|c| (n points of the subproblem).  A unit is then at most 2^-19 max|c|
+// for 1024 points, a rounding error near 1e-6 of the strengths, which the eps >= 1e-4 plans
+// that use it do not see.
+constexpr double kPackedTol = 1e-4;
+constexpr int kPackedBits = 30;
+
+// round(vr w) and round(vi w) packed, each rounded by one FP64 FMA with 1.5 2^52 (the integer
+// lands in the low mantissa bits, two's complement): the low word of the real part's pattern
+// is its integer mod 2^32, and the imaginary pattern minus the constant's is its integer.
+__device__ __forceinline__ unsigned long long pack_re_im(double vr, double vi, double w) {
+  const unsigned long long br = (unsigned long long)__double_as_longlong(__builtin_fma(vr, w, TileAcc<float>::kMagic));
+  const unsigned long long bi = (unsigned long long)__double_as_longlong(__builtin_fma(vi, w, TileAcc<float>::kMagic));
+  return (br << 32) + (bi - 0x4338000000000000ull);
+}
+
+__device__ __forceinline__ void unpack_re_im(unsigned long long w, long long& re, long long& im) {
+  im = (long long)(int)(unsigned)(w & 0xFFFFFFFFull);
+  re = (long long)(w - (unsigned long long)im) >> 32;
 }
 
 // compact record word of one coordinate (the layout of unpack_compact), tile-relative
@@ -393,15 +418,15 @@ __device__ __forceinline__ void bank_order(const SIArgs<float>& a, const int4& s
   __syncthreads();
 }
 
-// One block of the bank-spread kernel: subproblem sub of transform tr.
-template <typename CT, int D, int NS, bool REC, int C>
+// One block of the bank-spread kernel: subproblem sub of transform tr.  PK: packed tile.
+template <typename CT, int D, int NS, bool REC, int C, bool PK = false>
 __device__ __forceinline__ void spread_banked_block(const SIArgs<float>& a, int maxsub, int sub, int tr,
                                                     const cplx<float>* __restrict__ c, cplx<float>* __restrict__ fw) {
   using Word = unsigned long long;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Word* sre = reinterpret_cast<Word*>(smem_raw);
-  Word* sim = sre + a.tile_cells;
-  unsigned* sw = reinterpret_cast<unsigned*>(sim + a.tile_cells);
+  Word* sim = sre + (PK ? 0 : a.tile_cells);  // unused when packed
+  unsigned* sw = reinterpret_cast<unsigned*>(sre + (PK ? 1 : 2) * a.tile_cells);
   int* sp = reinterpret_cast<int*>(sw + (size_t)D * maxsub);
   unsigned short* order = reinterpret_cast<unsigned short*>(sp + maxsub);
   __shared__ int ccnt[C], cfill[C];
@@ -412,18 +437,19 @@ __device__ __forceinline__ void spread_banked_block(const SIArgs<float>& a, int 
   tile_origin<D>(sd, a.bs, a.nb, NS / 2, org);
   for (int64_t i = threadIdx.x; i < a.tile_cells; i += blockDim.x) {
     sre[i] = 0;
-    sim[i] = 0;
+    if constexpr (!PK) sim[i] = 0;
   }
   if (threadIdx.x < C) ccnt[threadIdx.x] = 0;
-  // fixed-point scale, as in k_spread_tiled
+  // fixed-point scale, as in k_spread_tiled (packed: 30 instead of 50 bits below 2^e)
+  constexpr int kBits = PK ? kPackedBits : 50;
   const double B = 2.0 * (double)n * (double)a.cmax[tr];
   const bool finite = isfinite(B);
   double scale = 1.0, inv_scale = 1.0;
   if (finite && B > 0.0) {
     int e;
     (void)frexp(B, &e);
-    scale = ldexp(1.0, 50 - e);
-    inv_scale = ldexp(1.0, e - 50);
+    scale = ldexp(1.0, kBits - e);
+    inv_scale = ldexp(1.0, e - kBits);
   }
   __syncthreads();
   bank_order<CT, D, NS, REC, C>(a, sd, org, maxsub, sw, sp, order, ccnt, cfill);
@@ -468,8 +494,12 @@ __device__ __forceinline__ void spread_banked_block(const SIArgs<float>& a, int 
         const int row = plane + ky * t0;
 #pragma unroll
         for (int kx = 0; kx < NS; ++kx) {
-          atomicAdd(sre + row + kx, TileAcc<float>::quant(vr, wd[kx]));
-          atomicAdd(sim + row + kx, TileAcc<float>::quant(vi, wd[kx]));
+          if constexpr (PK) {
+            atomicAdd(sre + row + kx, pack_re_im(vr, vi, wd[kx]));
+          } else {
+            atomicAdd(sre + row + kx, TileAcc<float>::quant(vr, wd[kx]));
+            atomicAdd(sim + row + kx, TileAcc<float>::quant(vi, wd[kx]));
+          }
         }
       }
     }
@@ -477,7 +507,13 @@ __device__ __forceinline__ void spread_banked_block(const SIArgs<float>& a, int 
   __syncthreads();
   cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
   for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x) {
-    const long long ir = (long long)sre[cell], ii = (long long)sim[cell];
+    long long ir, ii;
+    if constexpr (PK) {
+      unpack_re_im(sre[cell], ir, ii);
+    } else {
+      ir = (long long)sre[cell];
+      ii = (long long)sim[cell];
+    }
     if (finite && ir == 0 && ii == 0) continue;
     const float vr = finite ? (float)((double)ir * inv_scale) : NAN;
     const float vi = finite ? (float)((double)ii * inv_scale) : NAN;
@@ -487,10 +523,10 @@ __device__ __forceinline__ void spread_banked_block(const SIArgs<float>& a, int 
   }
 }
 
-template <typename CT, int D, int NS, bool REC, int C>
+template <typename CT, int D, int NS, bool REC, int C, bool PK = false>
 __global__ void __launch_bounds__(1024) k_spread_banked(SIArgs<float> a, int maxsub, const cplx<float>* __restrict__ c,
                                                         cplx<float>* __restrict__ fw) {
-  spread_banked_block<CT, D, NS, REC, C>(a, maxsub, (int)blockIdx.x, (int)blockIdx.y, c, fw);
+  spread_banked_block<CT, D, NS, REC, C, PK>(a, maxsub, (int)blockIdx.x, (int)blockIdx.y, c, fw);
 }
 
 // Several point sets (plans of one kernel size and dimension, any points and grids) in one

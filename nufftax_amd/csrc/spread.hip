@@ -42,13 +42,18 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
         // 2D/3D FP32: the bank-spread kernel when its point stage fits (maxsub <= 65535 points,
         // supports within 127 cells of the tile origin).  Tiles that leave room for one block
         // per CU get 1024 threads, the others 256 (several blocks per CU).
-        const size_t lb = banked_lds<D>(a.tile_cells, ps.maxsub);
+        // packed tiles (one atomic per support cell) for plans of eps >= 1e-4 with at most
+        // 1024 points per subproblem, the bound the packed scale assumes
+        const bool pk = ga.tol >= kPackedTol && ps.maxsub <= 1024 && !std::getenv("NUFFTX_NO_PACKED");
+        const size_t lb = banked_lds<D>(a.tile_cells, ps.maxsub, pk);
         bool fits = ps.maxsub <= 65535 && lb <= (size_t)lds_device_max_bytes();
         for (int d = 0; d < D; ++d) fits = fits && a.bs[d] <= 127;
         if (fits) {
-          auto kb = k_spread_banked<CT, D, NS, false, kBankClasses>;
+          auto kb = pk ? k_spread_banked<CT, D, NS, false, kBankClasses, true>
+                       : k_spread_banked<CT, D, NS, false, kBankClasses>;
           if constexpr (D == 2)
-            if (a.crec) kb = k_spread_banked<CT, D, NS, true, kBankClasses>;
+            if (a.crec)
+              kb = pk ? k_spread_banked<CT, D, NS, true, kBankClasses, true> : k_spread_banked<CT, D, NS, true, kBankClasses>;
           if (lb > 65536) (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
           const unsigned nt = 2 * lb > (size_t)lds_device_max_bytes() ? 1024 : 256;
           hipLaunchKernelGGL(kb, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(nt), lb, s, a, ps.maxsub,

@@ -194,6 +194,36 @@ def test_type1_vs_direct_c64(rng, dim, nm, isign):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("dim,nm", [(2, (48, 40)), (3, (20, 16, 12))])
+@pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4])
+@pytest.mark.parametrize("dist", ["uniform", "clustered"])
+def test_type1_packed_tiles_c64(rng, dim, nm, eps, dist, monkeypatch):
+    """complex64 type 1 at eps >= 1e-4 spreads into packed tiles (real and imaginary 32-bit
+    fixed point in one 64-bit word, a unit <= 2^-19 max|c| for 1024-point subproblems).
+    Bars: error vs the FP64 direct sum <= 2x the oracle's complex64 error + 2e-5, and packed
+    vs 64-bit-per-part tiles (NUFFTX_NO_PACKED) within 3e-5 relative l2."""
+    import nufftax_amd as N
+
+    M = 20000
+    if dist == "uniform":
+        coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(dim)]
+    else:
+        coords = [np.concatenate([rng.normal(0.3, 0.02, M - 500), rng.uniform(-np.pi, np.pi, 500)]) for _ in range(dim)]
+    coords = [x.astype(np.float32).astype(np.float64) for x in coords]
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64).astype(np.complex128)
+    fn = [N.nufft1d1, N.nufft2d1, N.nufft3d1][dim - 1]
+    args = [_t(x, torch.float32) for x in coords] + [_t(c, torch.complex64)]
+    packed = _np(fn(*args, nm, eps=eps))
+    monkeypatch.setenv("NUFFTX_NO_PACKED", "1")
+    full = _np(fn(*args, nm, eps=eps))
+    monkeypatch.delenv("NUFFTX_NO_PACKED")
+    assert R.rel_l2(packed, full) < 3e-5
+    truth = R.direct_type1(coords, c, nm, isign=1)
+    proxy = _proxy_err_type1(coords, c, nm, 1, eps)
+    err = R.rel_l2(packed, truth)
+    assert err <= 2 * proxy + 2e-5, (err, proxy)
+
+
 @pytest.mark.parametrize("dim,nm", [(1, (40,)), (2, (24, 18)), (3, (10, 12, 8))])
 @pytest.mark.parametrize("isign", [1, -1])
 def test_type2_vs_direct_c64(rng, dim, nm, isign):
