@@ -21,6 +21,21 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
   return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;
     if (tiled) {
+      if constexpr (std::is_same<T, float>::value && D >= 2 && NS <= 8) {
+        // column lanes pay where a point reads many cells (3D, ns = 8: 512 per point); in 2D and
+        // at smaller widths the per-point shuffles cost more than the fewer bank conflicts save
+        const char* ev = std::getenv("NUFFTX_INTERP_COLS");
+        const bool cols = ev ? std::atoi(ev) != 0 : (D == 3 && NS == 8);
+        if (cols) {
+          auto kc = k_gather_cols<CT, D, NS, false>;
+          if constexpr (D == 2)
+            if (a.crec) kc = k_gather_cols<CT, D, NS, true>;
+          if (lds > 65536) (void)hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+          hipLaunchKernelGGL(kc, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
+                             static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));
+          return hipGetLastError();
+        }
+      }
       auto kern = k_gather_tiled<T, CT, D, NS, false>;
       if constexpr (D == 2 && std::is_same<T, float>::value)
         if (a.crec) kern = k_gather_tiled<T, CT, D, NS, false, true>;

@@ -772,6 +772,96 @@ __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>
   gather_tiled_block<T, CT, D, NS, DERIV, REC>(a, fw, o, (int)blockIdx.x, (int)blockIdx.y);
 }
 
+// Values-only FP32 interpolation with eight lanes per point: lane kx of a point's group reads
+// column kx of the support (one cell per row, so a group reads 8 adjacent cells per step and a
+// wave step touches 8 runs of 64 bytes instead of 64 scattered cells), weights the rows in
+// registers, and the group sums its columns with three cross-lane steps.  Lane kx evaluates
+// weight kx of every dimension; the group shares them by shuffles.
+template <typename CT, int D, int NS, bool REC>
+__device__ __forceinline__ void gather_cols_block(const SIArgs<float>& a, const cplx<float>* __restrict__ fw,
+                                                  cplx<float>* __restrict__ out, int sub, int tr) {
+  static_assert(NS <= 8, "one lane per support column");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<float>* tile = reinterpret_cast<cplx<float>*>(smem_raw);
+  if (sub >= *a.nsub) return;
+  const int4 sd4 = a.subprob[sub];
+  int org[3];
+  tile_origin<D>(sd4, a.bs, a.nb, NS / 2, org);
+  const cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
+  for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x)
+    tile[cell] = fT[tile_cell_to_global<D>(cell, a.tile, org, a.nf)];
+  __syncthreads();
+  const int t0 = a.tile[0], t1 = a.tile[1];
+  const int lane = threadIdx.x & 63, kx = lane & 7, gbase = lane & ~7;
+  const bool col = kx < NS;
+  const int per = blockDim.x >> 3;  // points per block step
+  for (int j0 = 0; j0 < sd4.z; j0 += per) {
+    const int j = j0 + (int)(threadIdx.x >> 3);
+    const bool in = j < sd4.z;
+    int p = -1;
+    int l[3] = {0, 0, 0};
+    float t[3] = {0.0f, 0.0f, 0.0f};
+    if (in) {
+      p = a.perm[sd4.y + j];
+      if constexpr (REC) {
+        const uint2 rc = a.crec[sd4.y + j];
+        unpack_compact<float, NS>(rc.x, l[0], t[0]);
+        unpack_compact<float, NS>(rc.y, l[1], t[1]);
+      } else if ((unsigned)p < (unsigned)a.M) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const Folded f = fold_coord((double)static_cast<const CT*>(a.coord[d])[p], a.nf[d]);
+          const int i0 = support_start(f, NS / 2);
+          l[d] = i0 - org[d];
+          t[d] = (float)((double)i0 - f.xs);
+        }
+      }
+    }
+    const bool valid = in && (unsigned)p < (unsigned)a.M;
+    // weight kx of each dimension on lane kx, then every row / plane weight of the group
+    float wk[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) wk[d] = col ? es_phi(t[d] + (float)kx, a.es.beta_log2e, a.es.c) : 0.0f;
+    float wy[NS], wz[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      wy[k] = __shfl(wk[D >= 2 ? 1 : 0], gbase + k, 64);
+      wz[k] = D == 3 ? __shfl(wk[D - 1], gbase + k, 64) : 1.0f;
+    }
+    float sr = 0.0f, si = 0.0f;
+    if (valid && col) {
+      const int kzn = D == 3 ? NS : 1;
+#pragma unroll
+      for (int kz = 0; kz < kzn; ++kz) {
+        float pr = 0.0f, pi = 0.0f;
+        const cplx<float>* colp = tile + ((int64_t)(l[2] + kz) * t1 + l[1]) * t0 + l[0] + kx;
+#pragma unroll
+        for (int ky = 0; ky < NS; ++ky) {
+          const cplx<float> v = colp[(int64_t)ky * t0];
+          pr = fmaf(v.re, wy[ky], pr);
+          pi = fmaf(v.im, wy[ky], pi);
+        }
+        sr = fmaf(pr, wz[kz], sr);
+        si = fmaf(pi, wz[kz], si);
+      }
+      sr *= wk[0];
+      si *= wk[0];
+    }
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+      sr += __shfl_xor(sr, off, 64);
+      si += __shfl_xor(si, off, 64);
+    }
+    if (valid && kx == 0) out[(int64_t)tr * a.M + p] = cplx<float>{sr, si};
+  }
+}
+
+template <typename CT, int D, int NS, bool REC>
+__global__ void __launch_bounds__(256) k_gather_cols(SIArgs<float> a, const cplx<float>* __restrict__ fw,
+                                                     cplx<float>* __restrict__ out) {
+  gather_cols_block<CT, D, NS, REC>(a, fw, out, (int)blockIdx.x, (int)blockIdx.y);
+}
+
 // Values-only FP32 interpolation of several point sets in one launch (see SpreadMany).
 struct GatherMany {
   SIArgs<float> a[kMaxMany];
