@@ -36,6 +36,19 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
           return hipGetLastError();
         }
       }
+      if constexpr (D == 2 && std::is_same<T, float>::value && NS <= 8) {
+        // 16-byte tile reads (k_gather_rec_wide) unless NUFFTX_INTERP_WIDE=0; rows of an even
+        // number of cells keep the reads aligned
+        const char* ev = std::getenv("NUFFTX_INTERP_WIDE");
+        if (a.crec && (a.tile[0] % 2) == 0 && !(ev && std::atoi(ev) == 0)) {
+          const size_t lw = lds + 2 * sizeof(cplx<float>);
+          auto kw = k_gather_rec_wide<NS>;
+          if (lw > 65536) (void)hipFuncSetAttribute((const void*)kw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw);
+          hipLaunchKernelGGL(kw, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lw, s, a,
+                             static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));
+          return hipGetLastError();
+        }
+      }
       auto kern = k_gather_tiled<T, CT, D, NS, false>;
       if constexpr (D == 2 && std::is_same<T, float>::value)
         if (a.crec) kern = k_gather_tiled<T, CT, D, NS, false, true>;

@@ -772,6 +772,72 @@ __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>
   gather_tiled_block<T, CT, D, NS, DERIV, REC>(a, fw, o, (int)blockIdx.x, (int)blockIdx.y);
 }
 
+// 2D FP32 interpolation from compact records with 16-byte tile reads: a support row of NS cells
+// starting at l0 is read as NS/2 + 1 aligned cell pairs from l0 rounded down to even (tile rows
+// of an even number of cells keep every pair 16-byte aligned), with the x weights shifted by
+// one cell when l0 is odd and zero on the extra cell; two zero cells after the tile cover the
+// last row's overrun.
+template <int NS>
+__global__ void __launch_bounds__(256) k_gather_rec_wide(SIArgs<float> a, const cplx<float>* __restrict__ fw,
+                                                         cplx<float>* __restrict__ out) {
+  constexpr int NC = NS + 2;  // cells read per row
+  constexpr int NQ = NC / 2;  // 16-byte reads per row
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<float>* tile = reinterpret_cast<cplx<float>*>(smem_raw);
+  const int sub = blockIdx.x, tr = blockIdx.y;
+  if (sub >= *a.nsub) return;
+  const int4 sd4 = a.subprob[sub];
+  int org[3];
+  tile_origin<2>(sd4, a.bs, a.nb, NS / 2, org);
+  const cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
+  for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x)
+    tile[cell] = fT[tile_cell_to_global<2>(cell, a.tile, org, a.nf)];
+  if (threadIdx.x < 2) tile[a.tile_cells + threadIdx.x] = cplx<float>{0.0f, 0.0f};
+  __syncthreads();
+  const int t0 = a.tile[0];
+  const int hq = t0 >> 1;  // pairs per tile row
+  for (int j = threadIdx.x; j < sd4.z; j += blockDim.x) {
+    const uint2 rc = a.crec[sd4.y + j];
+    const int p = a.perm[sd4.y + j];
+    int l0, l1;
+    float tx, ty;
+    unpack_compact<float, NS>(rc.x, l0, tx);
+    unpack_compact<float, NS>(rc.y, l1, ty);
+    float wx[NS], wy[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      wx[k] = es_phi(tx + (float)k, a.es.beta_log2e, a.es.c);
+      wy[k] = es_phi(ty + (float)k, a.es.beta_log2e, a.es.c);
+    }
+    const bool odd = (l0 & 1) != 0;
+    float ws[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const float we = k < NS ? wx[k] : 0.0f;
+      const float wo = (k >= 1 && k <= NS) ? wx[k - 1] : 0.0f;
+      ws[k] = odd ? wo : we;
+    }
+    const float4* rowp = reinterpret_cast<const float4*>(tile + (int64_t)l1 * t0 + (l0 - (odd ? 1 : 0)));
+    float sr = 0.0f, si = 0.0f;
+#pragma unroll
+    for (int ky = 0; ky < NS; ++ky) {
+      const float4* q = rowp + (int64_t)ky * hq;
+      float rr = 0.0f, ri = 0.0f;
+#pragma unroll
+      for (int u = 0; u < NQ; ++u) {
+        const float4 v = q[u];
+        rr = fmaf(v.x, ws[2 * u], rr);
+        ri = fmaf(v.y, ws[2 * u], ri);
+        rr = fmaf(v.z, ws[2 * u + 1], rr);
+        ri = fmaf(v.w, ws[2 * u + 1], ri);
+      }
+      sr = fmaf(rr, wy[ky], sr);
+      si = fmaf(ri, wy[ky], si);
+    }
+    if ((unsigned)p < (unsigned)a.M) out[(int64_t)tr * a.M + p] = cplx<float>{sr, si};
+  }
+}
+
 // Values-only FP32 interpolation with eight lanes per point: lane kx of a point's group reads
 // column kx of the support (one cell per row, so a group reads 8 adjacent cells per step and a
 // wave step touches 8 runs of 64 bytes instead of 64 scattered cells), weights the rows in

@@ -466,6 +466,29 @@ def test_interp_f32_column_lanes(rng, dim, eps, cols, monkeypatch):
     assert R.rel_l2(_np(c), ref) < 2e-6
 
 
+@pytest.mark.parametrize("nfs", [(77, 64), (45, 27)])
+@pytest.mark.parametrize("eps", [1e-2, 1e-4, 1e-6])
+@pytest.mark.parametrize("wide", ["1", "0"])
+def test_interp2d_f32_wide_reads(rng, nfs, eps, wide, monkeypatch):
+    """2D FP32 interpolation from point records with 16-byte tile reads (default; tile rows of an
+    even cell count, odd and even support starts) and with 8-byte reads (NUFFTX_INTERP_WIDE=0),
+    and a grid whose tile rows have an odd cell count (8-byte path), vs the FP64 oracle, < 2e-6."""
+    from nufftax_amd._plan import F32, grid_plan
+
+    monkeypatch.setenv("NUFFTX_INTERP_WIDE", wide)
+    kp = R.compute_kernel_params(eps)
+    M = 20000
+    coords = [np.concatenate([rng.normal(-0.4, 0.05, M - 4000), rng.uniform(-np.pi, np.pi, 4000)]).astype(np.float32)
+              for _ in range(2)]
+    fw = (rng.standard_normal((2, nfs[1], nfs[0])) + 1j * rng.standard_normal((2, nfs[1], nfs[0]))).astype(np.complex64)
+    plan = grid_plan(2, nfs, kp.nspread, kp.beta, kp.c, 2, F32, torch.device(DEV))
+    plan.setpts([_t(x) for x in coords])
+    c = torch.empty((2, M), dtype=torch.complex64, device=DEV)
+    plan.interp(_t(fw), c)
+    ref = R.interp([x.astype(np.float64) for x in coords], fw.astype(np.complex128), kp)
+    assert R.rel_l2(_np(c), ref) < 2e-6
+
+
 # ------------------------------------------------------------------ fused FFT + deconvolution
 
 
