@@ -36,8 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # the same counters read for a 256 MiB copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
 # the summary with the commands is profiles/nufft2d_radial_4M_hbm_traffic.md.
 TRAFFIC_BYTES = {
-    "k_spread_banked<float,2,8,false,16>": 500970.6 * 1024 * 2.0 + 78768.4 * 1024,      # 1.11 GB
-    "k_gather_tiled<float,float,2,8,false,true>": 51764.0 * 1024 * 2.0 + 123569.1 * 1024,  # 0.23 GB
+    "k_spread_banked<float,2,8,false,16,false>": 501300.4 * 1024 * 2.0 + 78768.4 * 1024,  # 1.11 GB
+    "k_gather_rec_wide<8>": 51048.8 * 1024 * 2.0 + 123630.3 * 1024,                     # 0.23 GB
 }
 
 
@@ -313,10 +313,10 @@ def main():
     # 2D FP32 ns <= 8 runs on the matrix cores when NUFFTX_MFMA is set, else on the LDS-tiled kernels
     mfma = os.environ.get("NUFFTX_MFMA") is not None and os.environ.get("NUFFTX_NO_MFMA") is None
     if t_spread >= t_interp:
-        kname = "k_spread_mfma2d<8>" if mfma else "k_spread_banked<float,2,8,false,16>"
+        kname = "k_spread_mfma2d<8>" if mfma else "k_spread_banked<float,2,8,false,16,false>"
         kt, kb = t_spread, bytes_spread
     else:
-        kname = "k_interp_mfma2d<8>" if mfma else "k_gather_tiled<float,float,2,8,false,true>"
+        kname = "k_interp_mfma2d<8>" if mfma else "k_gather_rec_wide<8>"
         kt, kb = t_interp, bytes_interp
     achieved = kb / kt / 1e9
     traffic = None if mfma else TRAFFIC_BYTES.get(kname)
