@@ -36,8 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # the same counters read for a 256 MiB copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
 # the summary with the commands is profiles/nufft2d_radial_4M_hbm_traffic.md.
 TRAFFIC_BYTES = {
-    "k_spread_banked<float,2,8,false,16,false>": 501300.4 * 1024 * 2.0 + 78768.4 * 1024,  # 1.11 GB
-    "k_gather_rec_wide<8>": 51048.8 * 1024 * 2.0 + 123630.3 * 1024,                     # 0.23 GB
+    "k_spread_banked<float,2,8,false,16,false>": 501221.2 * 1024 * 2.0 + 78768.4 * 1024,  # 1.11 GB
+    "k_gather_wide<float,2,8,true>": 51039.2 * 1024 * 2.0 + 123658.1 * 1024,           # 0.23 GB
 }
 
 
@@ -316,7 +316,7 @@ def main():
         kname = "k_spread_mfma2d<8>" if mfma else "k_spread_banked<float,2,8,false,16,false>"
         kt, kb = t_spread, bytes_spread
     else:
-        kname = "k_interp_mfma2d<8>" if mfma else "k_gather_rec_wide<8>"
+        kname = "k_interp_mfma2d<8>" if mfma else "k_gather_wide<float,2,8,true>"
         kt, kb = t_interp, bytes_interp
     achieved = kb / kt / 1e9
     traffic = None if mfma else TRAFFIC_BYTES.get(kname)

@@ -21,28 +21,15 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
   return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;
     if (tiled) {
-      if constexpr (std::is_same<T, float>::value && D >= 2 && NS <= 8) {
-        // column lanes pay where a point reads many cells (3D, ns = 8: 512 per point); in 2D and
-        // at smaller widths the per-point shuffles cost more than the fewer bank conflicts save
-        const char* ev = std::getenv("NUFFTX_INTERP_COLS");
-        const bool cols = ev ? std::atoi(ev) != 0 : (D == 3 && NS == 8);
-        if (cols) {
-          auto kc = k_gather_cols<CT, D, NS, false>;
-          if constexpr (D == 2)
-            if (a.crec) kc = k_gather_cols<CT, D, NS, true>;
-          if (lds > 65536) (void)hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-          hipLaunchKernelGGL(kc, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
-                             static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));
-          return hipGetLastError();
-        }
-      }
-      if constexpr (D == 2 && std::is_same<T, float>::value && NS <= 8) {
-        // 16-byte tile reads (k_gather_rec_wide) unless NUFFTX_INTERP_WIDE=0; rows of an even
+      if constexpr (D >= 2 && std::is_same<T, float>::value && NS <= 8) {
+        // 16-byte tile reads (k_gather_wide) unless NUFFTX_INTERP_WIDE=0; rows of an even
         // number of cells keep the reads aligned
         const char* ev = std::getenv("NUFFTX_INTERP_WIDE");
-        if (a.crec && (a.tile[0] % 2) == 0 && !(ev && std::atoi(ev) == 0)) {
+        if ((a.tile[0] % 2) == 0 && !(ev && std::atoi(ev) == 0)) {
           const size_t lw = lds + 2 * sizeof(cplx<float>);
-          auto kw = k_gather_rec_wide<NS>;
+          auto kw = k_gather_wide<CT, D, NS, false>;
+          if constexpr (D == 2)
+            if (a.crec) kw = k_gather_wide<CT, D, NS, true>;
           if (lw > 65536) (void)hipFuncSetAttribute((const void*)kw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw);
           hipLaunchKernelGGL(kw, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lw, s, a,
                              static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));

@@ -772,14 +772,14 @@ __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>
   gather_tiled_block<T, CT, D, NS, DERIV, REC>(a, fw, o, (int)blockIdx.x, (int)blockIdx.y);
 }
 
-// 2D FP32 interpolation from compact records with 16-byte tile reads: a support row of NS cells
-// starting at l0 is read as NS/2 + 1 aligned cell pairs from l0 rounded down to even (tile rows
-// of an even number of cells keep every pair 16-byte aligned), with the x weights shifted by
-// one cell when l0 is odd and zero on the extra cell; two zero cells after the tile cover the
-// last row's overrun.
-template <int NS>
-__global__ void __launch_bounds__(256) k_gather_rec_wide(SIArgs<float> a, const cplx<float>* __restrict__ fw,
-                                                         cplx<float>* __restrict__ out) {
+// FP32 interpolation with 16-byte tile reads: a support row of NS cells starting at l0 is read
+// as NS/2 + 1 aligned cell pairs from l0 rounded down to even (tile rows of an even number of
+// cells keep every pair 16-byte aligned), with the x weights shifted by one cell when l0 is
+// odd and zero on the extra cell; two zero cells after the tile cover the last row's overrun.
+// REC (2D): points from the compact records; otherwise folded from the coordinates.
+template <typename CT, int D, int NS, bool REC>
+__global__ void __launch_bounds__(256) k_gather_wide(SIArgs<float> a, const cplx<float>* __restrict__ fw,
+                                                     cplx<float>* __restrict__ out) {
   constexpr int NC = NS + 2;  // cells read per row
   constexpr int NQ = NC / 2;  // 16-byte reads per row
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -788,28 +788,40 @@ __global__ void __launch_bounds__(256) k_gather_rec_wide(SIArgs<float> a, const 
   if (sub >= *a.nsub) return;
   const int4 sd4 = a.subprob[sub];
   int org[3];
-  tile_origin<2>(sd4, a.bs, a.nb, NS / 2, org);
+  tile_origin<D>(sd4, a.bs, a.nb, NS / 2, org);
   const cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
   for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x)
-    tile[cell] = fT[tile_cell_to_global<2>(cell, a.tile, org, a.nf)];
+    tile[cell] = fT[tile_cell_to_global<D>(cell, a.tile, org, a.nf)];
   if (threadIdx.x < 2) tile[a.tile_cells + threadIdx.x] = cplx<float>{0.0f, 0.0f};
   __syncthreads();
-  const int t0 = a.tile[0];
+  const int t0 = a.tile[0], t1 = a.tile[1];
   const int hq = t0 >> 1;  // pairs per tile row
   for (int j = threadIdx.x; j < sd4.z; j += blockDim.x) {
-    const uint2 rc = a.crec[sd4.y + j];
     const int p = a.perm[sd4.y + j];
-    int l0, l1;
-    float tx, ty;
-    unpack_compact<float, NS>(rc.x, l0, tx);
-    unpack_compact<float, NS>(rc.y, l1, ty);
-    float wx[NS], wy[NS];
+    int l[3] = {0, 0, 0};
+    float t[3] = {0.0f, 0.0f, 0.0f};
+    if constexpr (REC) {
+      const uint2 rc = a.crec[sd4.y + j];
+      unpack_compact<float, NS>(rc.x, l[0], t[0]);
+      unpack_compact<float, NS>(rc.y, l[1], t[1]);
+    } else {
+      if ((unsigned)p >= (unsigned)a.M) continue;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const Folded f = fold_coord((double)static_cast<const CT*>(a.coord[d])[p], a.nf[d]);
+        const int i0 = support_start(f, NS / 2);
+        l[d] = i0 - org[d];
+        t[d] = (float)((double)i0 - f.xs);
+      }
+    }
+    float wx[NS], wy[NS], wz[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      wx[k] = es_phi(tx + (float)k, a.es.beta_log2e, a.es.c);
-      wy[k] = es_phi(ty + (float)k, a.es.beta_log2e, a.es.c);
+      wx[k] = es_phi(t[0] + (float)k, a.es.beta_log2e, a.es.c);
+      wy[k] = es_phi(t[1] + (float)k, a.es.beta_log2e, a.es.c);
+      wz[k] = D == 3 ? es_phi(t[2] + (float)k, a.es.beta_log2e, a.es.c) : 1.0f;
     }
-    const bool odd = (l0 & 1) != 0;
+    const bool odd = (l[0] & 1) != 0;
     float ws[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -817,115 +829,34 @@ __global__ void __launch_bounds__(256) k_gather_rec_wide(SIArgs<float> a, const 
       const float wo = (k >= 1 && k <= NS) ? wx[k - 1] : 0.0f;
       ws[k] = odd ? wo : we;
     }
-    const float4* rowp = reinterpret_cast<const float4*>(tile + (int64_t)l1 * t0 + (l0 - (odd ? 1 : 0)));
+    const float4* base = reinterpret_cast<const float4*>(
+        tile + ((int64_t)l[2] * t1 + l[1]) * t0 + (l[0] - (odd ? 1 : 0)));
     float sr = 0.0f, si = 0.0f;
+    const int kzn = D == 3 ? NS : 1;
+#pragma unroll 1
+    for (int kz = 0; kz < kzn; ++kz) {
+      const float4* plane = base + (int64_t)kz * t1 * hq;
+      float pr = 0.0f, pi = 0.0f;
 #pragma unroll
-    for (int ky = 0; ky < NS; ++ky) {
-      const float4* q = rowp + (int64_t)ky * hq;
-      float rr = 0.0f, ri = 0.0f;
+      for (int ky = 0; ky < NS; ++ky) {
+        const float4* q = plane + (int64_t)ky * hq;
+        float rr = 0.0f, ri = 0.0f;
 #pragma unroll
-      for (int u = 0; u < NQ; ++u) {
-        const float4 v = q[u];
-        rr = fmaf(v.x, ws[2 * u], rr);
-        ri = fmaf(v.y, ws[2 * u], ri);
-        rr = fmaf(v.z, ws[2 * u + 1], rr);
-        ri = fmaf(v.w, ws[2 * u + 1], ri);
+        for (int u = 0; u < NQ; ++u) {
+          const float4 v = q[u];
+          rr = fmaf(v.x, ws[2 * u], rr);
+          ri = fmaf(v.y, ws[2 * u], ri);
+          rr = fmaf(v.z, ws[2 * u + 1], rr);
+          ri = fmaf(v.w, ws[2 * u + 1], ri);
+        }
+        pr = fmaf(rr, wy[ky], pr);
+        pi = fmaf(ri, wy[ky], pi);
       }
-      sr = fmaf(rr, wy[ky], sr);
-      si = fmaf(ri, wy[ky], si);
+      sr = fmaf(pr, wz[kz], sr);
+      si = fmaf(pi, wz[kz], si);
     }
     if ((unsigned)p < (unsigned)a.M) out[(int64_t)tr * a.M + p] = cplx<float>{sr, si};
   }
-}
-
-// Values-only FP32 interpolation with eight lanes per point: lane kx of a point's group reads
-// column kx of the support (one cell per row, so a group reads 8 adjacent cells per step and a
-// wave step touches 8 runs of 64 bytes instead of 64 scattered cells), weights the rows in
-// registers, and the group sums its columns with three cross-lane steps.  Lane kx evaluates
-// weight kx of every dimension; the group shares them by shuffles.
-template <typename CT, int D, int NS, bool REC>
-__device__ __forceinline__ void gather_cols_block(const SIArgs<float>& a, const cplx<float>* __restrict__ fw,
-                                                  cplx<float>* __restrict__ out, int sub, int tr) {
-  static_assert(NS <= 8, "one lane per support column");
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<float>* tile = reinterpret_cast<cplx<float>*>(smem_raw);
-  if (sub >= *a.nsub) return;
-  const int4 sd4 = a.subprob[sub];
-  int org[3];
-  tile_origin<D>(sd4, a.bs, a.nb, NS / 2, org);
-  const cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
-  for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x)
-    tile[cell] = fT[tile_cell_to_global<D>(cell, a.tile, org, a.nf)];
-  __syncthreads();
-  const int t0 = a.tile[0], t1 = a.tile[1];
-  const int lane = threadIdx.x & 63, kx = lane & 7, gbase = lane & ~7;
-  const bool col = kx < NS;
-  const int per = blockDim.x >> 3;  // points per block step
-  for (int j0 = 0; j0 < sd4.z; j0 += per) {
-    const int j = j0 + (int)(threadIdx.x >> 3);
-    const bool in = j < sd4.z;
-    int p = -1;
-    int l[3] = {0, 0, 0};
-    float t[3] = {0.0f, 0.0f, 0.0f};
-    if (in) {
-      p = a.perm[sd4.y + j];
-      if constexpr (REC) {
-        const uint2 rc = a.crec[sd4.y + j];
-        unpack_compact<float, NS>(rc.x, l[0], t[0]);
-        unpack_compact<float, NS>(rc.y, l[1], t[1]);
-      } else if ((unsigned)p < (unsigned)a.M) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const Folded f = fold_coord((double)static_cast<const CT*>(a.coord[d])[p], a.nf[d]);
-          const int i0 = support_start(f, NS / 2);
-          l[d] = i0 - org[d];
-          t[d] = (float)((double)i0 - f.xs);
-        }
-      }
-    }
-    const bool valid = in && (unsigned)p < (unsigned)a.M;
-    // weight kx of each dimension on lane kx, then every row / plane weight of the group
-    float wk[3];
-#pragma unroll
-    for (int d = 0; d < D; ++d) wk[d] = col ? es_phi(t[d] + (float)kx, a.es.beta_log2e, a.es.c) : 0.0f;
-    float wy[NS], wz[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      wy[k] = __shfl(wk[D >= 2 ? 1 : 0], gbase + k, 64);
-      wz[k] = D == 3 ? __shfl(wk[D - 1], gbase + k, 64) : 1.0f;
-    }
-    float sr = 0.0f, si = 0.0f;
-    if (valid && col) {
-      const int kzn = D == 3 ? NS : 1;
-#pragma unroll
-      for (int kz = 0; kz < kzn; ++kz) {
-        float pr = 0.0f, pi = 0.0f;
-        const cplx<float>* colp = tile + ((int64_t)(l[2] + kz) * t1 + l[1]) * t0 + l[0] + kx;
-#pragma unroll
-        for (int ky = 0; ky < NS; ++ky) {
-          const cplx<float> v = colp[(int64_t)ky * t0];
-          pr = fmaf(v.re, wy[ky], pr);
-          pi = fmaf(v.im, wy[ky], pi);
-        }
-        sr = fmaf(pr, wz[kz], sr);
-        si = fmaf(pi, wz[kz], si);
-      }
-      sr *= wk[0];
-      si *= wk[0];
-    }
-#pragma unroll
-    for (int off = 4; off > 0; off >>= 1) {
-      sr += __shfl_xor(sr, off, 64);
-      si += __shfl_xor(si, off, 64);
-    }
-    if (valid && kx == 0) out[(int64_t)tr * a.M + p] = cplx<float>{sr, si};
-  }
-}
-
-template <typename CT, int D, int NS, bool REC>
-__global__ void __launch_bounds__(256) k_gather_cols(SIArgs<float> a, const cplx<float>* __restrict__ fw,
-                                                     cplx<float>* __restrict__ out) {
-  gather_cols_block<CT, D, NS, REC>(a, fw, out, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // Values-only FP32 interpolation of several point sets in one launch (see SpreadMany).

@@ -442,23 +442,22 @@ def test_interp2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
     assert R.rel_l2(_np(c), ref) < 2e-6
 
 
-@pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("eps", [1e-2, 1e-4, 1e-6])
-@pytest.mark.parametrize("cols", ["1", "0"])
-def test_interp_f32_column_lanes(rng, dim, eps, cols, monkeypatch):
-    """FP32 interpolation with eight lanes per point (NUFFTX_INTERP_COLS=1; the default for 3D at
-    ns = 8) and with one lane per point (=0) vs the FP64 oracle, rel l2 < 2e-6, two transforms."""
+@pytest.mark.parametrize("wide", ["1", "0"])
+def test_interp3d_f32_wide_reads(rng, eps, wide, monkeypatch):
+    """3D FP32 interpolation with 16-byte tile reads (default) and 8-byte reads
+    (NUFFTX_INTERP_WIDE=0) vs the FP64 oracle, rel l2 < 2e-6, two transforms."""
     from nufftax_amd._plan import F32, grid_plan
 
-    monkeypatch.setenv("NUFFTX_INTERP_COLS", cols)
+    monkeypatch.setenv("NUFFTX_INTERP_WIDE", wide)
     kp = R.compute_kernel_params(eps)
-    nfs = (77, 64) if dim == 2 else (41, 36, 34)
+    nfs = (41, 36, 34)
     M = 20000
     coords = [np.concatenate([rng.normal(0.3, 0.05, M - 3000), rng.uniform(-np.pi, np.pi, 3000)]).astype(np.float32)
-              for _ in range(dim)]
+              for _ in range(3)]
     shape = (2,) + tuple(reversed(nfs))
     fw = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
-    plan = grid_plan(dim, nfs, kp.nspread, kp.beta, kp.c, 2, F32, torch.device(DEV))
+    plan = grid_plan(3, nfs, kp.nspread, kp.beta, kp.c, 2, F32, torch.device(DEV))
     plan.setpts([_t(x) for x in coords])
     c = torch.empty((2, M), dtype=torch.complex64, device=DEV)
     plan.interp(_t(fw), c)
@@ -472,7 +471,8 @@ def test_interp_f32_column_lanes(rng, dim, eps, cols, monkeypatch):
 def test_interp2d_f32_wide_reads(rng, nfs, eps, wide, monkeypatch):
     """2D FP32 interpolation from point records with 16-byte tile reads (default; tile rows of an
     even cell count, odd and even support starts) and with 8-byte reads (NUFFTX_INTERP_WIDE=0),
-    and a grid whose tile rows have an odd cell count (8-byte path), vs the FP64 oracle, < 2e-6."""
+    and a grid whose tile rows have an odd cell count at ns = 4 (8-byte path), vs the FP64
+    oracle, < 2e-6."""
     from nufftax_amd._plan import F32, grid_plan
 
     monkeypatch.setenv("NUFFTX_INTERP_WIDE", wide)
