@@ -1,6 +1,8 @@
 // Gather with ES-kernel-derivative weights: coordinate gradients of spread and
 // interp (nufftax/core/spread.py:820-1305) and the fused value+coordinate
 // backward of the type-1/type-2 transforms.  Kernels in si_kernels.hpp.
+#include <cstdlib>
+
 #include "si_kernels.hpp"
 
 namespace nufftx {
@@ -21,6 +23,18 @@ hipError_t grad_impl(const GridOpArgs& ga, const void* G, const void* P, void* g
   return dispatch_ns(ga.g->ns, [&](auto nsc) -> hipError_t {
     constexpr int NS = decltype(nsc)::value;
     if (tiled) {
+      if constexpr (std::is_same<T, float>::value && D >= 2 && NS <= 8) {
+        // 16-byte tile reads, as in the value interpolation (NUFFTX_INTERP_WIDE=0: 8-byte)
+        const char* ev = std::getenv("NUFFTX_INTERP_WIDE");
+        if ((a.tile[0] % 2) == 0 && !(ev && std::atoi(ev) == 0)) {
+          const size_t lw = lds + 2 * sizeof(cplx<float>);
+          auto kw = k_gather_wide_deriv<CT, D, NS>;
+          if (lw > 65536) (void)hipFuncSetAttribute((const void*)kw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw);
+          hipLaunchKernelGGL(kw, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lw, s, a,
+                             static_cast<const cplx<float>*>(G), o);
+          return hipGetLastError();
+        }
+      }
       auto kern = k_gather_tiled<T, CT, D, NS, true>;
       if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,

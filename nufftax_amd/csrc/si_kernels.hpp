@@ -859,6 +859,94 @@ __global__ void __launch_bounds__(256) k_gather_wide(SIArgs<float> a, const cplx
   }
 }
 
+// The coordinate-gradient gather (values, derivative sums; emit_point applies P and the chain
+// factor) with the 16-byte pair reads of k_gather_wide: FP32, points folded from coordinates.
+template <typename CT, int D, int NS>
+__global__ void __launch_bounds__(256) k_gather_wide_deriv(SIArgs<float> a, const cplx<float>* __restrict__ fw,
+                                                           GatherOut<float, D, true> o) {
+  constexpr int NC = NS + 2;
+  constexpr int NQ = NC / 2;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<float>* tile = reinterpret_cast<cplx<float>*>(smem_raw);
+  const int sub = blockIdx.x, tr = blockIdx.y;
+  if (sub >= *a.nsub) return;
+  const int4 sd4 = a.subprob[sub];
+  int org[3];
+  tile_origin<D>(sd4, a.bs, a.nb, NS / 2, org);
+  const cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
+  for (int64_t cell = threadIdx.x; cell < a.tile_cells; cell += blockDim.x)
+    tile[cell] = fT[tile_cell_to_global<D>(cell, a.tile, org, a.nf)];
+  if (threadIdx.x < 2) tile[a.tile_cells + threadIdx.x] = cplx<float>{0.0f, 0.0f};
+  __syncthreads();
+  const int t0 = a.tile[0], t1 = a.tile[1];
+  const int hq = t0 >> 1;
+  for (int j = threadIdx.x; j < sd4.z; j += blockDim.x) {
+    const int p = a.perm[sd4.y + j];
+    if ((unsigned)p >= (unsigned)a.M) continue;
+    float w[D][NS], dw[D][NS];
+    int i0[D];
+    point_weights_deriv<float, CT, D, NS>(a, p, w, dw, i0);
+    int l[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) l[d] = i0[d] - org[d];
+    const bool odd = (l[0] & 1) != 0;
+    float ws[NC], dws[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const bool ine = k < NS, ino = k >= 1 && k <= NS;
+      ws[k] = odd ? (ino ? w[0][k - 1] : 0.0f) : (ine ? w[0][k] : 0.0f);
+      dws[k] = odd ? (ino ? dw[0][k - 1] : 0.0f) : (ine ? dw[0][k] : 0.0f);
+    }
+    const float4* base = reinterpret_cast<const float4*>(
+        tile + ((int64_t)l[2] * t1 + l[1]) * t0 + (l[0] - (odd ? 1 : 0)));
+    cplx<float> s0 = {0.0f, 0.0f}, sd[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) sd[d] = {0.0f, 0.0f};
+    const int kzn = D == 3 ? NS : 1;
+#pragma unroll 1
+    for (int kz = 0; kz < kzn; ++kz) {
+      const float4* plane = base + (int64_t)kz * t1 * hq;
+      cplx<float> p0 = {0.0f, 0.0f}, px = {0.0f, 0.0f}, py = {0.0f, 0.0f};
+#pragma unroll
+      for (int ky = 0; ky < NS; ++ky) {
+        const float4* q = plane + (int64_t)ky * hq;
+        cplx<float> r0 = {0.0f, 0.0f}, rx = {0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+          const float4 v = q[u];
+          r0.re = fmaf(v.x, ws[2 * u], fmaf(v.z, ws[2 * u + 1], r0.re));
+          r0.im = fmaf(v.y, ws[2 * u], fmaf(v.w, ws[2 * u + 1], r0.im));
+          rx.re = fmaf(v.x, dws[2 * u], fmaf(v.z, dws[2 * u + 1], rx.re));
+          rx.im = fmaf(v.y, dws[2 * u], fmaf(v.w, dws[2 * u + 1], rx.im));
+        }
+        const float wy = w[1][ky], dwy = dw[1][ky];
+        p0.re = fmaf(r0.re, wy, p0.re);
+        p0.im = fmaf(r0.im, wy, p0.im);
+        px.re = fmaf(rx.re, wy, px.re);
+        px.im = fmaf(rx.im, wy, px.im);
+        py.re = fmaf(r0.re, dwy, py.re);
+        py.im = fmaf(r0.im, dwy, py.im);
+      }
+      if constexpr (D == 2) {
+        s0 = p0;
+        sd[0] = px;
+        sd[1] = py;
+      } else {
+        const float wz = w[D - 1][kz], dwz = dw[D - 1][kz];
+        s0.re = fmaf(p0.re, wz, s0.re);
+        s0.im = fmaf(p0.im, wz, s0.im);
+        sd[0].re = fmaf(px.re, wz, sd[0].re);
+        sd[0].im = fmaf(px.im, wz, sd[0].im);
+        sd[1].re = fmaf(py.re, wz, sd[1].re);
+        sd[1].im = fmaf(py.im, wz, sd[1].im);
+        sd[D - 1].re = fmaf(p0.re, dwz, sd[D - 1].re);
+        sd[D - 1].im = fmaf(p0.im, dwz, sd[D - 1].im);
+      }
+    }
+    emit_point<float, D, true>(a, o, tr, p, s0, sd);
+  }
+}
+
 // Values-only FP32 interpolation of several point sets in one launch (see SpreadMany).
 struct GatherMany {
   SIArgs<float> a[kMaxMany];

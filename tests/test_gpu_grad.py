@@ -228,3 +228,39 @@ def test_vmap_fast_path_and_fallback(rng):
     out2 = vmap(lambda xx, cc: N.nufft2d1(xx, y, cc, nm))(xb, cs[:3])
     loop2 = torch.stack([N.nufft2d1(xb[i], y, cs[i], nm) for i in range(3)])
     assert R.rel_l2(_np(out2), _np(loop2)) < 1e-12
+
+
+@pytest.mark.parametrize("dim,nm", [(2, (40, 36)), (3, (16, 12, 10))])
+@pytest.mark.parametrize("kind", [1, 2])
+def test_c64_coordinate_grads_wide_reads_match(rng, dim, nm, kind, monkeypatch):
+    """complex64 value and coordinate gradients through the gather with 16-byte tile reads
+    (default) equal those of the 8-byte-read gather (NUFFTX_INTERP_WIDE=0) to 1e-5 relative:
+    the two differ only in FP32 summation order."""
+    import nufftax_amd as N
+
+    M = 6000
+    xs0 = [np.concatenate([rng.normal(0.5, 0.1, M - 1000), rng.uniform(-np.pi, np.pi, 1000)]) for _ in range(dim)]
+    if kind == 1:
+        d0 = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+        fn = [None, N.nufft2d1, N.nufft3d1][dim - 1]
+    else:
+        shp = tuple(reversed(nm))
+        d0 = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        fn = [None, N.nufft2d2, N.nufft3d2][dim - 1]
+    g0 = None
+
+    def grads(wide):
+        nonlocal g0
+        monkeypatch.setenv("NUFFTX_INTERP_WIDE", wide)
+        xs = [_t(x, torch.float32).requires_grad_(True) for x in xs0]
+        d = _t(d0, torch.complex64).requires_grad_(True)
+        out = fn(*xs, d, nm, eps=1e-6) if kind == 1 else fn(*xs, d, eps=1e-6)
+        if g0 is None:
+            g0 = torch.randn(out.shape, dtype=out.dtype, device=out.device)
+        torch.sum((out * g0.conj()).real).backward()
+        return [_np(t.grad) for t in xs + [d]]
+
+    a = grads("1")
+    b = grads("0")
+    for u, v in zip(a, b):
+        assert R.rel_l2(u, v) < 1e-5
