@@ -41,7 +41,9 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
       if constexpr (D >= 2 && std::is_same<T, float>::value) {
         // 2D/3D FP32: the bank-spread kernel when its point stage fits (maxsub <= 65535 points,
         // supports within 127 cells of the tile origin).  Tiles that leave room for one block
-        // per CU get 1024 threads, the others 256 (several blocks per CU).
+        // per CU get 1024 threads, the others 512 (several blocks per CU; measured on the 2D
+        // radial workload: 211-214 us against 224-230 us with 256; NUFFTX_SPREAD_THREADS
+        // overrides).
         // packed tiles (one atomic per support cell) for plans of eps >= 1e-4 with at most
         // 1024 points per subproblem, the bound the packed scale assumes
         const bool pk = ga.tol >= kPackedTol && ps.maxsub <= 1024 && !std::getenv("NUFFTX_NO_PACKED");
@@ -55,7 +57,9 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
             if (a.crec)
               kb = pk ? k_spread_banked<CT, D, NS, true, kBankClasses, true> : k_spread_banked<CT, D, NS, true, kBankClasses>;
           if (lb > 65536) (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
-          const unsigned nt = 2 * lb > (size_t)lds_device_max_bytes() ? 1024 : 256;
+          unsigned nt = 2 * lb > (size_t)lds_device_max_bytes() ? 1024 : 512;
+          if (const char* e = std::getenv("NUFFTX_SPREAD_THREADS"))
+            nt = (unsigned)std::min(1024, std::max(64, std::atoi(e) / 64 * 64));
           hipLaunchKernelGGL(kb, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(nt), lb, s, a, ps.maxsub,
                              static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
           return hipGetLastError();
