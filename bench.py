@@ -36,8 +36,8 @@ HBM_PEAK_GBS = 8000.0
 # the same counters read for a 256 MiB copy (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
 # the summary with the commands is profiles/nufft2d_radial_4M_hbm_traffic.md.
 TRAFFIC_BYTES = {
-    "k_spread_banked<float,2,8,false,16,false>": 501221.2 * 1024 * 2.0 + 78768.4 * 1024,  # 1.11 GB
-    "k_gather_wide<float,2,8,true>": 51039.2 * 1024 * 2.0 + 123658.1 * 1024,           # 0.23 GB
+    "k_spread_banked<float,2,8,false,16,false>": 499800.5 * 1024 * 2.0 + 78768.4 * 1024,  # 1.11 GB
+    "k_gather_wide<float,2,8,true>": 51023.6 * 1024 * 2.0 + 123608.3 * 1024,           # 0.23 GB
 }
 
 
