@@ -1,15 +1,23 @@
 """Python handle around a nufftx plan, plus a small per-stream plan cache.
 
-A plan owns its device buffers (sort state, fine grid, rocFFT plans).  Points
-are set once per coordinate set: the cache remembers which tensors (and which
-in-place versions of them) were last sorted and skips a repeated setpts.  The
-plan keeps those tensors alive because execute re-reads coordinates through
-the sorted permutation.
+A plan owns its device buffers (sort state, fine grid, rocFFT plans).  The
+public transforms sort their points on every call, so a result depends only on
+the arguments of that call (the reference's transforms are pure functions,
+nufftax/transforms/nufft1.py:92-162).  Keeping a sort resident across calls is
+an explicit opt-in (``setpts(..., reuse=True)``, used by callers that own the
+coordinate tensors, such as the serving loop's registered point sets).  The
+plan keeps the last coordinate tensors alive because execute re-reads
+coordinates through the sorted permutation.
+
+Threads: the cache is locked, and a plan is used by one thread at a time
+(``with plan.lock:`` around setpts + execute); plans are per stream, so work of
+one plan stays in stream order.
 """
 
 from __future__ import annotations
 
 import ctypes
+import threading
 from collections import OrderedDict
 
 import torch
@@ -42,6 +50,7 @@ class Plan:
         self._pts_refs = None
         self.M = 0
         self.N = 0
+        self.lock = threading.RLock()
 
     # ---------------------------------------------------------------- creation
     @classmethod
@@ -77,12 +86,15 @@ class Plan:
     def _sync_stream(self):
         check(lib().nufftx_set_stream(self.h, ctypes.c_void_p(_stream_handle(self.device))), "nufftx_set_stream")
 
-    def setpts(self, coords, targets=None):
+    def setpts(self, coords, targets=None, reuse=False):
+        """Fold and bin-sort the points.  reuse=True skips the sort when the same tensors (same
+        storage, in-place version, shape and dtype) were the last ones sorted: only for callers
+        that guarantee the coordinates were not written behind the version counter since."""
         coords = list(coords)
         targets = list(targets) if targets is not None else []
         key = (tuple(_tensor_key(t) for t in coords), tuple(_tensor_key(t) for t in targets))
         self._sync_stream()
-        if key == self._pts_key:
+        if reuse and key == self._pts_key:
             return
         M = coords[0].numel()
         N = targets[0].numel() if targets else 0
@@ -178,23 +190,31 @@ class PlanCache:
     def __init__(self, capacity: int = 24):
         self.capacity = capacity
         self._plans: OrderedDict = OrderedDict()
+        self._lock = threading.Lock()
 
     def get(self, key, factory):
-        p = self._plans.get(key)
-        if p is not None:
-            self._plans.move_to_end(key)
-            return p
-        p = factory()
-        self._plans[key] = p
-        while len(self._plans) > self.capacity:
-            _, old = self._plans.popitem(last=False)
-            old.destroy()
+        with self._lock:
+            p = self._plans.get(key)
+            if p is not None:
+                self._plans.move_to_end(key)
+                return p
+            p = factory()
+            self._plans[key] = p
+            evicted = []
+            while len(self._plans) > self.capacity:
+                evicted.append(self._plans.popitem(last=False)[1])
+        for old in evicted:
+            with old.lock:
+                old.destroy()
         return p
 
     def clear(self):
-        while self._plans:
-            _, p = self._plans.popitem()
-            p.destroy()
+        with self._lock:
+            plans = list(self._plans.values())
+            self._plans.clear()
+        for p in plans:
+            with p.lock:
+                p.destroy()
 
 
 CACHE = PlanCache()

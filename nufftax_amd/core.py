@@ -181,11 +181,9 @@ class _GridCtx(NamedTuple):
     params: KernelParams
 
 
-def _plan_for(ctx: _GridCtx, n_trans, prec, dev, coords):
+def _plan_for(ctx: _GridCtx, n_trans, prec, dev):
     kp = ctx.params
-    plan = grid_plan(ctx.dim, ctx.nf, kp.nspread, kp.beta, kp.c, n_trans, prec, dev)
-    plan.setpts(coords)
-    return plan
+    return grid_plan(ctx.dim, ctx.nf, kp.nspread, kp.beta, kp.c, n_trans, prec, dev)
 
 
 def _norm_coords(coords, rdt, dev):
@@ -210,8 +208,10 @@ def _raw_spread(ctx: _GridCtx, coords, c):
     fw = torch.empty((n_trans,) + tuple(reversed(ctx.nf)), dtype=cdt, device=dev)
     if n_trans == 0:
         return fw
-    plan = _plan_for(ctx, n_trans, prec, dev, xs)
-    plan.spread(c.contiguous(), fw)
+    plan = _plan_for(ctx, n_trans, prec, dev)
+    with plan.lock:
+        plan.setpts(xs)
+        plan.spread(c.contiguous(), fw)
     return fw
 
 
@@ -224,8 +224,10 @@ def _raw_interp(ctx: _GridCtx, coords, fw):
     c = torch.empty((n_trans, M), dtype=cdt, device=dev)
     if n_trans == 0 or M == 0:
         return c.zero_()
-    plan = _plan_for(ctx, n_trans, prec, dev, xs)
-    plan.interp(fw.contiguous(), c)
+    plan = _plan_for(ctx, n_trans, prec, dev)
+    with plan.lock:
+        plan.setpts(xs)
+        plan.interp(fw.contiguous(), c)
     return c
 
 
@@ -238,8 +240,10 @@ def _raw_coord_grad(ctx: _GridCtx, coords, G, P):
     grads = [torch.empty(M, dtype=rdt, device=dev) for _ in range(ctx.dim)]
     if n_trans == 0 or M == 0:
         return [g.zero_() for g in grads]
-    plan = _plan_for(ctx, n_trans, prec, dev, xs)
-    plan.gather_grad(G.contiguous(), P.to(cdt).contiguous(), None, grads)
+    plan = _plan_for(ctx, n_trans, prec, dev)
+    with plan.lock:
+        plan.setpts(xs)
+        plan.gather_grad(G.contiguous(), P.to(cdt).contiguous(), None, grads)
     return grads
 
 

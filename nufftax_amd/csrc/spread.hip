@@ -3,6 +3,7 @@
 #include <cstdlib>
 
 #include "si_mfma.hpp"
+#include "si_win.hpp"
 
 namespace nufftx {
 
@@ -107,11 +108,38 @@ bool spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, h
   return false;
 }
 
+// The matrix-core window spread (2D FP32, ns <= 8, subproblems of <= 1024 points).
+template <typename CT>
+bool spread_win(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hipError_t& err) {
+  const PointSet& ps = *ga.ps;
+  if (!win2d_applicable(*ga.g, ga.prec, ps.maxsub) || !use_win_spread()) return false;
+  SIArgs<float> a = make_si_args<float>(ga);
+  auto launch = [&](auto nsc) -> bool {
+    constexpr int NS = decltype(nsc)::value;
+    const size_t lds = win_spread_lds<NS>(ps.maxsub);
+    if (lds + 1024 > (size_t)lds_device_max_bytes()) return false;
+    auto kern = a.crec ? k_spread_win2d<CT, NS, true> : k_spread_win2d<CT, NS, false>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kWinThreads), lds, s, a,
+                       static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
+    err = hipGetLastError();
+    return true;
+  };
+  switch (ga.g->ns) {
+    case 2: return launch(std::integral_constant<int, 2>{});
+    case 4: return launch(std::integral_constant<int, 4>{});
+    case 6: return launch(std::integral_constant<int, 6>{});
+    case 8: return launch(std::integral_constant<int, 8>{});
+  }
+  return false;
+}
+
 template <typename T, typename CT>
 hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     hipError_t err = hipSuccess;
     if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma() && spread_mfma(ga, c, fw, s, err)) return err;
+    if (spread_win<CT>(ga, c, fw, s, err)) return err;
   }
   switch (ga.g->dim) {
     case 1: return spread_impl<T, CT, 1>(ga, c, fw, s);

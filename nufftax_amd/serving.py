@@ -55,6 +55,7 @@ class ServeRequest:
     isign: int
     future: Future = field(default_factory=Future)
     ready: object = None         # HIP event recorded on the submitter's stream
+    consumer: object = None      # the submitter's stream, which reads the result
     stream_index: int | None = None  # pin to one of the server's streams (plan warm-up)
     nbytes: int = 0
     t_submit: float = 0.0
@@ -153,8 +154,9 @@ class NufftServer:
         req.stream_index = None if stream_index is None else int(stream_index) % len(self.streams)
         if req.nbytes > self.budget:
             raise ValueError(f"request needs {req.nbytes} B of HBM, over the server budget of {self.budget} B")
+        req.consumer = torch.cuda.current_stream(self.device)
         req.ready = torch.cuda.Event()
-        req.ready.record(torch.cuda.current_stream(self.device))
+        req.ready.record(req.consumer)
         req.future.request = req
         req.t_submit = time.perf_counter()
         self._q.put(req)
@@ -348,6 +350,11 @@ class NufftServer:
             self._release(nbytes)
             for g, o in zip(group, outs):
                 g.t_done = t
+                # the result was allocated on a server stream: tell the caching allocator that
+                # the submitter's stream uses it, so that its memory is not handed to a later
+                # batch while work the submitter queued on the result may still be reading it
+                if g.consumer is not None:
+                    o.record_stream(g.consumer)
                 g.future.set_result(o)
 
 

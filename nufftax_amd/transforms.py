@@ -142,8 +142,9 @@ def _raw_type1(meta: _Meta, c: torch.Tensor, coords, isign=None, upsampfac=None)
     out = torch.empty((n_trans,) + tuple(reversed(meta.n_modes)), dtype=cdt, device=dev)
     for s, e in _chunks(n_trans):
         plan = transform_plan(1, dim, meta.n_modes, e - s, meta.eps, isign, uf, prec, dev)
-        plan.setpts(xs)
-        plan.execute(cc[s:e], out[s:e], isign)
+        with plan.lock:
+            plan.setpts(xs)
+            plan.execute(cc[s:e], out[s:e], isign)
     return out.reshape(lead + tuple(reversed(meta.n_modes))).to(c.dtype)
 
 
@@ -162,8 +163,9 @@ def _raw_type2(meta: _Meta, f: torch.Tensor, coords, isign=None, upsampfac=None)
     out = torch.empty((n_trans, M), dtype=cdt, device=dev)
     for s, e in _chunks(n_trans):
         plan = transform_plan(2, dim, nm, e - s, meta.eps, isign, uf, prec, dev)
-        plan.setpts(xs)
-        plan.execute(out[s:e], ff[s:e], isign)
+        with plan.lock:
+            plan.setpts(xs)
+            plan.execute(out[s:e], ff[s:e], isign)
     return out.reshape(lead + (M,)).to(f.dtype)
 
 
@@ -182,8 +184,9 @@ def _raw_type3(meta: _Meta, c: torch.Tensor, coords, targets, isign=None):
     out = torch.empty((n_trans, N), dtype=cdt, device=dev)
     for s, e in _chunks(n_trans):
         plan = transform_plan(3, dim, meta.n_modes, e - s, meta.eps, isign, meta.upsampfac, prec, dev)
-        plan.setpts(xs, ss)
-        plan.execute(cc[s:e], out[s:e])
+        with plan.lock:
+            plan.setpts(xs, ss)
+            plan.execute(cc[s:e], out[s:e])
     return out.reshape(lead + (N,)).to(c.dtype)
 
 
@@ -212,9 +215,10 @@ def _vjp_type12(kind, meta: _Meta, src, coords, g, need_src, need_coord):
     gcoord = [torch.zeros(M, dtype=rdt, device=dev) if need_coord[d] else None for d in range(dim)]
     for s, e in _chunks(n_trans):
         plan = transform_plan(kind, dim, n_modes, e - s, meta.eps, meta.isign, meta.upsampfac_T, prec, dev)
-        plan.setpts(xs)
         part = [torch.empty(M, dtype=rdt, device=dev) if gc is not None else None for gc in gcoord]
-        plan.vjp(ss[s:e], gg[s:e], grad_src[s:e] if grad_src is not None else None, part)
+        with plan.lock:
+            plan.setpts(xs)
+            plan.vjp(ss[s:e], gg[s:e], grad_src[s:e] if grad_src is not None else None, part)
         for d in range(dim):
             if gcoord[d] is not None:
                 gcoord[d] += part[d]

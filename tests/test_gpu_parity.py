@@ -389,6 +389,37 @@ def test_type3_vs_direct(rng, dim, prec):
         assert err < 1e-7
 
 
+@pytest.mark.parametrize("dim", [1, 2])
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_type3_small_grids_vs_direct(rng, dim, prec):
+    """Grid sizes below 2 * nspread, as compute_type3_grid_size returns them for a small
+    space-bandwidth product (x in [-pi, pi], s in [-1, 1]: nf = 12 at eps 1e-6, 10 at 1e-4);
+    the reference accepts any n_modes (nufft3.py:266)."""
+    import nufftax_amd as N
+
+    M, Nt = 3000, 700
+    for eps in (1e-4, 1e-6):
+        xs = [rng.uniform(-np.pi, np.pi, M) for _ in range(dim)]
+        ss = [rng.uniform(-1, 1, Nt) for _ in range(dim)]
+        c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+        rd, cd = (torch.float32, torch.complex64) if prec == "c64" else (torch.float64, torch.complex128)
+        if prec == "c64":
+            xs = [x.astype(np.float32).astype(np.float64) for x in xs]
+            ss = [s.astype(np.float32).astype(np.float64) for s in ss]
+            c = c.astype(np.complex64).astype(np.complex128)
+        nfs = tuple(N.compute_type3_grid_size(xs[d], ss[d], eps=eps) for d in range(dim))
+        ns = N.compute_kernel_params(eps).nspread
+        assert all(n < 2 * ns for n in nfs), nfs
+        if dim == 1:
+            out = N.nufft1d3(_t(xs[0], rd), _t(c, cd), _t(ss[0], rd), nfs[0], eps=eps)
+        else:
+            out = N.nufft2d3(_t(xs[0], rd), _t(xs[1], rd), _t(c, cd), _t(ss[0], rd), _t(ss[1], rd), nfs, eps=eps)
+        truth = R.direct_type3(xs, c, ss)
+        err = R.rel_l2(_np(out), truth)
+        proxy = R.rel_l2(R.nufft_type3(xs, c, ss, nfs, eps=eps), truth)  # FP64 restatement, same grid
+        assert err <= 2 * proxy + 10 * eps, (eps, nfs, err, proxy)
+
+
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
 @pytest.mark.parametrize("mfma", ["1", None])
