@@ -58,6 +58,14 @@ struct PointSet {
   int64_t cap_crec = 0;
   int32_t* sub_off = nullptr;    // max_subprob * 17 sub-bin starts per subproblem
   float* cmax = nullptr;          // n_trans: max |re|, |im| of the strengths (matrix-core spread)
+  // 2D FP32 point sets spread by the matrix-core window kernel: the inverse permutation
+  // (slot[i] = sorted position of point i, written by the sort) and, per execute, every
+  // transform's points packed in sorted order as {x word, y word, re c, im c} (compact
+  // words as in crec), so that the spread reads its points contiguously
+  int32_t* slot = nullptr;       // M
+  bool has_slot = false;
+  uint4* packed = nullptr;       // n_trans * M
+  int64_t cap_slot = 0, cap_packed = 0;
   int64_t cap_recs = 0, cap_sub_off = 0, cap_cmax = 0;
   int64_t cap_M = 0, cap_keys = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0, cap_sub_start = 0;
   std::vector<void*> retired;    // outgrown buffers, freed by pointset_free
@@ -69,6 +77,15 @@ struct SortTuning {
   int maxsub = 1024;           // points per subproblem
 };
 
+// Whether a geometry takes the matrix-core window spread (si_win.hpp): 2D FP32, ns <= 8
+// (window 16 = SB + ns with SB >= 8), square bins of 4 x 4 sub-bins, subproblems of at most
+// 1024 points (two per thread of a 512-thread block)
+inline bool win2d_geometry(const Geometry& g, int prec, int maxsub) {
+  if (g.dim != 2 || prec != 0 || g.ns > 8 || maxsub > 1024) return false;
+  const int bs = 4 * (16 - g.ns);
+  return g.bs[0] == bs && g.bs[1] == bs && g.tile[0] == bs + g.ns && g.tile[1] == bs + g.ns;
+}
+
 // Tile size / LDS budget helpers
 void choose_bins(Geometry& g, int prec, int lds_budget);
 int lds_budget_bytes();
@@ -78,8 +95,13 @@ int lds_device_max_bytes();
 // rec_kind (2D only): also write point records, 1 = full (matrix-core kernels),
 // 2 = compact (LDS-tiled kernels); sub_sb: sub-bin side of the matrix-core
 // geometry (16 - ns) to record sub-bins in full records, 0 otherwise
+// want_slot (2D): also write the inverse permutation, and size the packed records for n_trans
+// transforms (n_trans 0: left to the first spread)
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
-                         int rec_kind = 0, int sub_sb = 0);
+                         int rec_kind = 0, int sub_sb = 0, bool want_slot = false, int64_t n_trans = 1);
+// packed[t][slot[i]] = {x word, y word, c[t][i]} of a 2D point set sorted with want_slot (FP32
+// strengths; a non-finite coordinate makes its strength NaN)
+hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s);
 
 // Device buffer of at least `need` elements (contents not kept when it grows).  The
 // outgrown buffer is not freed here: hipFree waits for the whole device, which would

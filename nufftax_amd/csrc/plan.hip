@@ -349,13 +349,17 @@ int check_plan(nufftx_plan p) {
 }
 
 // interp: the point set is interpolated to (type 2, grid operators, type-3 targets).
-int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g, bool interp) {
+int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g, bool interp, bool spread) {
   std::string err;
   const bool subsort = subsort_applicable(p->tune, g, p->prec);
   // records pay for their cost in the sort only where the point set is interpolated;
   // the matrix-core kernels need the full records with sub-bins
   const int rec_kind = g.dim == 2 && p->prec == kF32 ? (subsort ? 1 : interp ? 2 : 0) : 0;
-  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, rec_kind, subsort ? 16 - g.ns : 0);
+  // point sets that the matrix-core window kernel spreads get the inverse permutation (and,
+  // when spreading is their forward use, the packed-record buffer)
+  const bool slot = win2d_geometry(g, p->prec, p->tune.maxsub);
+  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, rec_kind, subsort ? 16 - g.ns : 0, slot,
+                               spread ? p->n_trans : 0);
   if (e != hipSuccess) {
     if (!err.empty()) return fail(NUFFTX_ERR_INVALID_ARG, err);
     return fail(NUFFTX_ERR_HIP, std::string("bin-sort: ") + hipGetErrorString(e));
@@ -620,7 +624,7 @@ int nufftx_setpts(nufftx_plan p, int64_t M, const void* x, const void* y, const 
     p->ps.M = M;
     for (int d = 0; d < 3; ++d) p->ps.coord[d] = d < p->dim ? xs[d] : nullptr;
     p->ps.coord_f64 = p->prec == kF64;
-    if (int r = sort_points(p, p->ps, p->g, p->type != 1)) return r;
+    if (int r = sort_points(p, p->ps, p->g, p->type != 1, p->type != 2)) return r;
     p->pts_set = true;
     return NUFFTX_OK;
   }
@@ -655,8 +659,8 @@ int nufftx_setpts(nufftx_plan p, int64_t M, const void* x, const void* y, const 
     p->pt.coord[d] = d < p->dim ? p->sr[d] : nullptr;
   }
   p->ps.coord_f64 = p->pt.coord_f64 = true;
-  if ((r = sort_points(p, p->ps, p->g, false))) return r;
-  if ((r = sort_points(p, p->pt, p->gi, true))) return r;
+  if ((r = sort_points(p, p->ps, p->g, false, true))) return r;
+  if ((r = sort_points(p, p->pt, p->gi, true, false))) return r;
   p->pts_set = true;
   return NUFFTX_OK;
 }

@@ -9,23 +9,38 @@
 // 4 points of one sub-bin is one v_mfma_f32_16x16x4f32 per part (re, im) into a
 // window accumulator held in registers; no per-point LDS atomics.
 //
-// Block = one subproblem (<= 2 * blockDim points of one bin):
-//   1. stage: every point's support start gives its sub-bin; the points are counted
-//      per sub-bin (LDS), and written into LDS in sub-bin order, each sub-bin padded to
-//      a multiple of 4 with zero-strength entries, as {Zx, Zy, re c, im c} with
-//      Z = round((i0 - xs - o) 2^23) the support offset relative to the sub-bin's window
-//      (o = support start - window start, in [0, SB]) in fixed point;
+// The points come packed in sorted order, {x word, y word, re c, im c} per point and
+// transform (pointset_pack2d), word = l << 25 | u: l the support start in the bin's tile,
+// u = round((t + ns/2 + 1) 2^24), t = i0 - xs from the FP64 fold.
+//
+// Block = one subproblem (<= 1024 points of one bin), 8 waves:
+//   1. order: every point's support start gives its sub-bin; the points are ranked per
+//      sub-bin inside each wave (four ballots) and their indices written into LDS in
+//      sub-bin order (stable), each sub-bin padded to a multiple of 4 with
+//      zero-strength entries; the block also takes max(|re c|, |im c|);
 //   2. accumulate: the groups of 4 are split into equal contiguous ranges, one per
-//      wave.  Lane (i, k) of a group evaluates wy_k at window row i and wx_k at window
-//      column i directly (z = (Z + i 2^23) 2^-23, ES kernel in registers, zero outside
-//      the support) and feeds A[i][k] = c_k wy_k(i), B[k][i] = wx_k(i).  When the
-//      wave's walk leaves a sub-bin, the window is added into the block's tile in
-//      exact 64-bit fixed point (integer LDS atomics, 4 rows x 16 cells per step);
+//      wave, walked in batches of 16 points.  Per batch, the wave re-reads the batch's
+//      records (cache-resident) and each lane evaluates ns/2 kernel weights of one
+//      dimension of one point (ES kernel in registers), written into entries 16 - ns ..
+//      15 of an LDS row of 17 floats whose other entries stay zero: the weight of window
+//      row (or column) i of a point with window offset o is entry 16 - ns + i - o of its
+//      row, read straight on into the next row's zeros (i - o runs over
+//      [-(16 - ns), 15]), so the MFMA operands A[i][k] = c_k wy_k(i), B[k][i] = wx_k(i)
+//      are two LDS reads and two multiplies per lane, no bounds checks.  When the wave's
+//      walk leaves a sub-bin, the window is added into the block's tile in fixed point
+//      (integer LDS atomics, 4 rows x 16 cells per step);
 //   3. flush: the tile's non-zero cells go to the fine grid with FP32 global atomics.
-// The fixed-point scale is 2^(50 - e) with 2^e above 2 n max(|re c|, |im c|) over the
-// subproblem's n points (a block maximum taken while staging): every window value and
-// partial tile sum stays below 2^50.  Windows accumulate in FP32 (like the reference's
-// segment_sum); the tile sum of the windows is exact.
+// Windows accumulate in FP32 (like the reference's segment_sum).  Tiles (PK = false): two
+// 64-bit words per cell, scale 2^(50 - e) with 2^e above 2 n max(|re c|, |im c|) over the
+// subproblem's n points, so every partial sum stays below 2^50 and the tile sum of the
+// windows is exact.  Packed tiles (PK = true, the default): one 64-bit word per cell, the
+// real part in its high and the imaginary part in its low 32 bits, scale 2^(30 - e) with
+// 2^e above S = sum_j max(|re c_j|, |im c_j|), which bounds every window value and every
+// final tile value of either part: the word adds modulo 2^64, so only the final sums must
+// fit 31 bits (they do, with a factor 2 to spare) and partial sums may wrap.  A window value
+// is then rounded to S 2^-30 (about 2^-20 of the subproblem's strength sum), 4 adds per cell
+// and subproblem on average -- near FP32 rounding -- for half the tile and half the
+// atomics, which lets four blocks share a CU.
 //
 // MFMA operand layout (v_mfma_f32_16x16x4f32, 64 lanes):
 //   A[i][k] in lane i + 16 k,  B[k][j] in lane j + 16 k,  D[i][j] in lane j + 16 (i / 4), element i % 4.
@@ -40,8 +55,12 @@ namespace nufftx {
 typedef float win_f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWinThreads = 512;  // 8 waves per subproblem
-constexpr int kWinPpt = 2;        // points per thread while staging: subproblems of <= 1024 points
-constexpr double kWinZScale = 8388608.0;  // 2^23
+constexpr int kWinPpt = 2;        // points per thread while ordering: subproblems of <= 1024 points
+constexpr int kWinBatch = 16;     // points per weight batch of a wave
+constexpr int kWinRow = 17;       // LDS row stride (floats): odd, so that the rows of a batch fall on distinct banks
+constexpr int kWinRowF = kWinBatch * kWinRow + 16;       // floats of one row block (+ trailing zeros)
+constexpr int kWinWaveF = 2 * kWinRowF + 4 * kWinBatch;  // floats per wave: x rows, y rows, meta
+static_assert(kWinPpt * kWinThreads == 1024, "win2d_geometry admits subproblems of up to 1024 points");
 
 template <int NS>
 struct WinGeom {
@@ -50,176 +69,254 @@ struct WinGeom {
   static constexpr int T = BS + NS;   // tile side
 };
 
-// LDS bytes of the window spread for subproblems of maxsub points
-template <int NS>
+__host__ __device__ inline size_t win_order_bytes(int maxsub) { return ((size_t)(maxsub + 3 * 16) * sizeof(uint16_t) + 15) / 16 * 16; }
+
+// LDS bytes of the window spread (dynamic part) for subproblems of maxsub points
+template <int NS, bool PK>
 inline size_t win_spread_lds(int maxsub) {
   constexpr int T = WinGeom<NS>::T;
-  return 2 * (size_t)T * T * sizeof(unsigned long long) + (size_t)(maxsub + 3 * 16) * sizeof(float4);
+  return (PK ? 1 : 2) * (size_t)T * T * sizeof(unsigned long long) + win_order_bytes(maxsub) +
+         (size_t)(kWinThreads / 64) * kWinWaveF * sizeof(float);
 }
 
-// ES kernel at z' = z 2^23 (c23 = c 2^-46): the same function as es_phi, zero outside the support
-__device__ __forceinline__ float es_phi_scaled(float zu, float beta_log2e, float c23) {
-  const float cz2 = c23 * zu * zu;
-  float a = 1.0f - cz2;
-  a = a > 0.0f ? a : 0.0f;
-  const float s = __builtin_sqrtf(a);
-  const float v = __builtin_amdgcn_exp2f(-beta_log2e * cz2 * __builtin_amdgcn_rcpf(1.0f + s));
-  return cz2 <= 1.0f ? v : 0.0f;
+// Orders a wave's LDS writes before its later reads of other lanes' data (and those
+// reads before the next writes).
+__device__ __forceinline__ void wave_sync_win() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename CT, int NS, bool REC>
-__global__ void __launch_bounds__(kWinThreads) k_spread_win2d(SIArgs<float> a, const cplx<float>* __restrict__ c,
+template <int NS, bool PK>
+__global__ void __launch_bounds__(kWinThreads) k_spread_win2d(SIArgs<float> a, int maxsub,
+                                                              const uint4* __restrict__ packed,
                                                               cplx<float>* __restrict__ fw) {
   using G = WinGeom<NS>;
   using Word = unsigned long long;
   constexpr int SB = G::SB, T = G::T, TC = T * T;
+  constexpr int W = kWinThreads / 64;
+  constexpr uint16_t kPad = 0xFFFF;  // order entry of a zero-strength padding point
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Word* sre = reinterpret_cast<Word*>(smem_raw);
-  Word* sim = sre + TC;
-  int4* stage = reinterpret_cast<int4*>(sim + TC);
-  __shared__ int cnt[16], fill[16], start[17];
+  Word* sim = sre + TC;  // unused when packed
+  uint16_t* order = reinterpret_cast<uint16_t*>(sre + (PK ? 1 : 2) * TC);
+  __shared__ int wcnt[kWinPpt * W * 16];  // per (pass, wave, sub-bin): count, then first slot
+  __shared__ int cnt[16], start[17];
   __shared__ unsigned smax;
-  __shared__ int sbad;
+  __shared__ float ssum[W];
   const int sub = blockIdx.x;
   if (sub >= *a.nsub) return;
   const int tr = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int4 sd = a.subprob[sub];
   const int n = sd.z;
+  const uint4* __restrict__ pk = packed + (int64_t)tr * a.M + sd.y;
+  float* const lf = reinterpret_cast<float*>(smem_raw);
+  const int wof = (int)(((PK ? 1 : 2) * (size_t)TC * sizeof(Word) + win_order_bytes(maxsub)) / 4) + wave * kWinWaveF;
+  float* rowx = lf + wof;         // x weights: row q at 17 q, weights at 17 q + 16 - NS .. 17 q + 15
+  float* rowy = rowx + kWinRowF;  // y weights, same layout
+  int4* meta = reinterpret_cast<int4*>(rowy + kWinRowF);  // {y read index, x read index, re c, im c}
   int org[3];
   tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
   for (int e = tid; e < TC; e += kWinThreads) {
     sre[e] = 0;
-    sim[e] = 0;
+    if constexpr (!PK) sim[e] = 0;
   }
-  if (tid < 16) cnt[tid] = 0;
-  if (tid == 0) {
-    smax = 0u;
-    sbad = 0;
-  }
+  for (int e = lane; e < kWinRowF; e += 64)  // the zero parts of the weight rows stay zero
+    if (e % kWinRow < 16 - NS || e % kWinRow == 16 || e >= kWinBatch * kWinRow) {
+      rowx[e] = 0.0f;
+      rowy[e] = 0.0f;
+    }
+  if (lane < kWinBatch)  // read indices inside the wave's rows until the first batch sets them
+    meta[lane] = make_int4(wof + kWinRowF + kWinRow * lane, wof + kWinRow * lane, 0, 0);
+  for (int e = tid; e < kWinPpt * W * 16; e += kWinThreads) wcnt[e] = 0;
+  if (tid == 0) smax = 0u;
   __syncthreads();
 
-  // ---- 1. stage
-  const cplx<float>* cT = c + (int64_t)tr * a.M;
-  int4 rec[kWinPpt];
-  int sb[kWinPpt];
+  // ---- 1. order
+  int sb[kWinPpt], rank[kWinPpt];
   unsigned mymax = 0;
-  bool bad = false;
+  float mysum = 0.0f;
+  uint4 pr[kWinPpt];
+#pragma unroll
+  for (int r = 0; r < kWinPpt; ++r) {  // all loads first
+    const int j = tid + r * kWinThreads;
+    pr[r] = j < n ? pk[j] : make_uint4(0u, 0u, 0u, 0u);
+  }
 #pragma unroll
   for (int r = 0; r < kWinPpt; ++r) {
     const int j = tid + r * kWinThreads;
     sb[r] = -1;
     if (j >= n) continue;
-    const int p = a.perm[sd.y + j];
-    if ((unsigned)p >= (unsigned)a.M) continue;
-    int l[2];
-    double t[2];
-    if constexpr (REC) {
-      const uint2 w = a.crec[sd.y + j];
-      l[0] = (int)(w.x >> 25);
-      l[1] = (int)(w.y >> 25);
-      t[0] = (double)(w.x & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1);
-      t[1] = (double)(w.y & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1);
-    } else {
+    const uint4 p = pr[r];
+    const int sx = min((int)(p.x >> 25) / SB, 3), sy = min((int)(p.y >> 25) / SB, 3);
+    const float are = fabsf(__uint_as_float(p.z)), aim = fabsf(__uint_as_float(p.w));
+    mymax = max(mymax, max(__float_as_uint(are), __float_as_uint(aim)));
+    mysum += fmaxf(are, aim) + 0.0f * (are + aim);  // NaN in either part propagates
+    sb[r] = sy * 4 + sx;
+  }
+  // rank of each point among the wave's points of its sub-bin (match by four ballots)
 #pragma unroll
-      for (int d = 0; d < 2; ++d) {
-        const Folded f = fold_coord((double)static_cast<const CT*>(a.coord[d])[p], a.nf[d]);
-        const int i0 = support_start(f, NS / 2);
-        l[d] = i0 - org[d];
-        t[d] = (double)i0 - f.xs;
-      }
-    }
-    int s2[2], Z[2];
+  for (int r = 0; r < kWinPpt; ++r) {
+    const bool valid = sb[r] >= 0;
+    uint64_t m = __ballot(valid);
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      s2[d] = min(max(l[d], 0) / SB, 3);
-      const double z = (t[d] - (double)(l[d] - s2[d] * SB)) * kWinZScale;
-      bad = bad || !(fabs(z) < 1e9);  // non-finite coordinate
-      Z[d] = bad ? 0 : (int)rint(z);
+    for (int b = 0; b < 4; ++b) {
+      const uint64_t bal = __ballot(valid && ((sb[r] >> b) & 1));
+      m &= ((sb[r] >> b) & 1) ? bal : ~bal;
     }
-    const cplx<float> cv = cT[p];
-    mymax = max(mymax, max(__float_as_uint(fabsf(cv.re)), __float_as_uint(fabsf(cv.im))));
-    rec[r] = make_int4(Z[0], Z[1], __float_as_int(cv.re), __float_as_int(cv.im));
-    sb[r] = s2[1] * 4 + s2[0];
-    atomicAdd(&cnt[sb[r]], 1);
+    rank[r] = popc_below(m);
+    if (valid && rank[r] == 0) wcnt[(r * W + wave) * 16 + sb[r]] = (int)__popcll(m);
   }
   // block maximum of |re c|, |im c| (non-negative floats order as their bit patterns;
   // a NaN pattern is above every finite one)
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) mymax = max(mymax, (unsigned)__shfl_xor((int)mymax, off, 64));
-  if (lane == 0) atomicMax(&smax, mymax);
-  if (bad) sbad = 1;
+  for (int off = 32; off > 0; off >>= 1) {
+    mymax = max(mymax, (unsigned)__shfl_xor((int)mymax, off, 64));
+    mysum += __shfl_xor(mysum, off, 64);
+  }
+  if (lane == 0) {
+    atomicMax(&smax, mymax);
+    ssum[wave] = mysum;
+  }
+  __syncthreads();
+  if (tid < 16) {  // per sub-bin count
+    int c = 0;
+    for (int e = 0; e < kWinPpt * W; ++e) c += wcnt[e * 16 + tid];
+    cnt[tid] = c;
+  }
   __syncthreads();
   if (tid == 0) {  // padded sub-bin starts (multiples of 4)
     int acc = 0;
     for (int s = 0; s < 16; ++s) {
       start[s] = acc;
-      fill[s] = acc;
       acc += (cnt[s] + 3) & ~3;
     }
     start[16] = acc;
   }
   __syncthreads();
-#pragma unroll
-  for (int r = 0; r < kWinPpt; ++r) {
-    if (sb[r] < 0) continue;
-    const int slot = atomicAdd(&fill[sb[r]], 1);
-    stage[slot] = rec[r];
+  if (tid < 16) {  // first slot of every (pass, wave) run of the sub-bin
+    int run = start[tid];
+    for (int e = 0; e < kWinPpt * W; ++e) {
+      const int v = wcnt[e * 16 + tid];
+      wcnt[e * 16 + tid] = run;
+      run += v;
+    }
   }
-  // zero-strength padding entries: z at the window origin, c = 0
-  if (tid < 16 * 3) {
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kWinPpt; ++r)
+    if (sb[r] >= 0) order[wcnt[(r * W + wave) * 16 + sb[r]] + rank[r]] = (uint16_t)(tid + r * kWinThreads);
+  if (tid < 16 * 3) {  // zero-strength padding entries
     const int s = tid / 3, q = tid % 3;
     const int slot = start[s] + cnt[s] + q;
-    if (slot < start[s + 1]) stage[slot] = make_int4(0, 0, 0, 0);
+    if (slot < start[s + 1]) order[slot] = kPad;
   }
   const float cm = __uint_as_float(smax);
-  const double B = 2.0 * (double)n * (double)cm;
-  const bool finite = isfinite(B) && sbad == 0;
+  double B;
+  if constexpr (PK) {
+    float S = 0.0f;
+#pragma unroll
+    for (int w = 0; w < W; ++w) S += ssum[w];
+    B = (double)S * (1.0 + 1.0 / 1024.0) + (double)cm;  // above the exact sum despite FP32 rounding
+  } else {
+    B = 2.0 * (double)n * (double)cm;
+  }
+  const bool finite = isfinite(B) && isfinite(cm);
+  constexpr int kBits = PK ? 30 : 50;
   double scale = 1.0, inv_scale = 1.0;
   if (finite && B > 0.0) {
     int e;
     (void)frexp(B, &e);  // B < 2^e
-    scale = ldexp(1.0, 50 - e);
-    inv_scale = ldexp(1.0, e - 50);
+    scale = ldexp(1.0, kBits - e);
+    inv_scale = ldexp(1.0, e - kBits);
   }
   __syncthreads();
 
   // ---- 2. accumulate
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  constexpr int W = kWinThreads / 64;
   const int NG = start[16] >> 2;
   const int g0 = (int)(((int64_t)NG * wave) / W), g1 = (int)(((int64_t)NG * (wave + 1)) / W);
   const int i = lane & 15, k = lane >> 4;
-  const int ioff = i << 23;
-  const float bl2e = a.es.beta_log2e;
-  const float c23 = (float)((double)a.es.c / (kWinZScale * kWinZScale));
+  // weight staging: lane -> (point sq, dimension sdim, weights sh NS/2 .. sh NS/2 + NS/2 - 1)
+  const int sq = lane & (kWinBatch - 1), sdim = (lane >> 4) & 1, sh = lane >> 5;
+  const float bl2e = a.es.beta_log2e, kc = a.es.c;
+  float* const myrow = (sdim ? rowy : rowx) + kWinRow * sq + (16 - NS) + sh * (NS / 2);
+  const int ry0 = wof + kWinRowF + (16 - NS), rx0 = wof + (16 - NS);  // first weight of row 0 (y, x)
   win_f4 acc_re = {0.f, 0.f, 0.f, 0.f}, acc_im = {0.f, 0.f, 0.f, 0.f};
   int s = 0;
   while (s < 15 && start[s + 1] <= 4 * g0) ++s;
+  int next = start[s + 1];
   auto flush = [&](int sbin) {
     const int x = (sbin & 3) * SB + i;
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
       const int cell = ((sbin >> 2) * SB + 4 * k + rr) * T + x;
-      if (acc_re[rr] != 0.0f) atomicAdd(&sre[cell], TileAcc<float>::quant((double)acc_re[rr], scale));
-      if (acc_im[rr] != 0.0f) atomicAdd(&sim[cell], TileAcc<float>::quant((double)acc_im[rr], scale));
+      if constexpr (PK) {
+        if (acc_re[rr] != 0.0f || acc_im[rr] != 0.0f)
+          atomicAdd(&sre[cell], pack_re_im((double)acc_re[rr], (double)acc_im[rr], scale));
+      } else {
+        if (acc_re[rr] != 0.0f) atomicAdd(&sre[cell], TileAcc<float>::quant((double)acc_re[rr], scale));
+        if (acc_im[rr] != 0.0f) atomicAdd(&sim[cell], TileAcc<float>::quant((double)acc_im[rr], scale));
+      }
     }
   };
-  for (int g = g0; g < g1 && finite; ++g) {
-    if (4 * g >= start[s + 1]) {
-      flush(s);
-      acc_re = win_f4{0.f, 0.f, 0.f, 0.f};
-      acc_im = win_f4{0.f, 0.f, 0.f, 0.f};
-      do {
-        ++s;
-      } while (s < 15 && start[s + 1] <= 4 * g);
+  // the records of the next batch are loaded while the current one is processed
+  auto fetch = [&](int gb) {
+    const int q = 4 * gb + sq;
+    const int j = q < 4 * g1 ? order[q] : kPad;
+    return j != kPad ? pk[j] : make_uint4(0u, 0u, 0u, 0u);
+  };
+  uint4 p = fetch(g0);
+  for (int gb = g0; gb < g1 && finite; gb += kWinBatch / 4) {
+    // -- weights of the batch's points
+    const int q = 4 * gb + sq;
+    if (q < 4 * g1) {
+      const unsigned wd = sdim ? p.y : p.x;
+      const float t = (float)((double)(wd & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1));
+#pragma unroll
+      for (int m = 0; m < NS / 2; ++m) myrow[m] = es_phi(t + (float)(sh * (NS / 2) + m), bl2e, kc);
+      if (lane < kWinBatch) {
+        const int lx = (int)(p.x >> 25), ly = (int)(p.y >> 25);
+        const int ox = lx - min(lx / SB, 3) * SB, oy = ly - min(ly / SB, 3) * SB;  // offsets in the window
+        // LDS float index of entry 16 - ns - o of the point's rows (a lane reads it + i)
+        meta[sq] = make_int4(ry0 + kWinRow * sq - oy, rx0 + kWinRow * sq - ox, (int)p.z, (int)p.w);
+      }
     }
-    const int4 rc = stage[4 * g + k];
-    const float wy = es_phi_scaled((float)(rc.y + ioff), bl2e, c23);
-    const float wx = es_phi_scaled((float)(rc.x + ioff), bl2e, c23);
-    const float are = __int_as_float(rc.z) * wy, aim = __int_as_float(rc.w) * wy;
-    acc_re = __builtin_amdgcn_mfma_f32_16x16x4f32(are, wx, acc_re, 0, 0, 0);
-    acc_im = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, wx, acc_im, 0, 0, 0);
+    // the next batch's records load while this batch's groups run
+    p = fetch(gb + kWinBatch / 4);
+    wave_sync_win();
+    // -- one MFMA pair per group of 4 points; all operands of the batch are read first
+    // (entries past the wave's range hold stale but in-range read indices)
+    constexpr int GB = kWinBatch / 4;
+    const int ng = min(g1 - gb, GB);
+    int4 mt[GB];
+    float wy[GB], wx[GB];
+#pragma unroll
+    for (int u = 0; u < GB; ++u) mt[u] = meta[4 * u + k];
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      wy[u] = lf[mt[u].x + i];
+      wx[u] = lf[mt[u].y + i];
+    }
+#pragma unroll
+    for (int u = 0; u < GB; ++u) {
+      if (u >= ng) break;
+      const int g = gb + u;
+      if (4 * g >= next) {
+        flush(s);
+        acc_re = win_f4{0.f, 0.f, 0.f, 0.f};
+        acc_im = win_f4{0.f, 0.f, 0.f, 0.f};
+        do {
+          ++s;
+        } while (s < 15 && start[s + 1] <= 4 * g);
+        next = start[s + 1];
+      }
+      const float are = __int_as_float(mt[u].z) * wy[u], aim = __int_as_float(mt[u].w) * wy[u];
+      acc_re = __builtin_amdgcn_mfma_f32_16x16x4f32(are, wx[u], acc_re, 0, 0, 0);
+      acc_im = __builtin_amdgcn_mfma_f32_16x16x4f32(aim, wx[u], acc_im, 0, 0, 0);
+    }
+    wave_sync_win();
   }
   if (g1 > g0 && finite) flush(s);
   __syncthreads();
@@ -228,7 +325,13 @@ __global__ void __launch_bounds__(kWinThreads) k_spread_win2d(SIArgs<float> a, c
   cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
   const int tile3[3] = {T, T, 1};
   for (int cell = tid; cell < TC; cell += kWinThreads) {
-    const long long ir = (long long)sre[cell], ii = (long long)sim[cell];
+    long long ir, ii;
+    if constexpr (PK) {
+      unpack_re_im(sre[cell], ir, ii);
+    } else {
+      ir = (long long)sre[cell];
+      ii = (long long)sim[cell];
+    }
     if (finite && ir == 0 && ii == 0) continue;
     const float vr = finite ? (float)((double)ir * inv_scale) : NAN;
     const float vi = finite ? (float)((double)ii * inv_scale) : NAN;
@@ -238,18 +341,19 @@ __global__ void __launch_bounds__(kWinThreads) k_spread_win2d(SIArgs<float> a, c
   }
 }
 
-// NUFFTX_SPREAD_WIN=1 selects the window spread where it applies
-inline bool use_win_spread() {
-  const char* e = std::getenv("NUFFTX_SPREAD_WIN");
+// NUFFTX_WIN_EXACT=1: two 64-bit words per tile cell (exact tile sums) instead of packed tiles
+inline bool win_exact_tiles() {
+  const char* e = std::getenv("NUFFTX_WIN_EXACT");
   return e != nullptr && e[0] == '1';
 }
 
-// Whether the window spread applies to a geometry: 2D FP32, ns <= 8 (window 16 = SB + ns
-// with SB >= 8), square bins of 4 x 4 sub-bins, subproblems of at most 2 points per thread.
-inline bool win2d_applicable(const Geometry& g, int prec, int maxsub) {
-  if (g.dim != 2 || prec != kF32 || g.ns > 8 || maxsub > kWinPpt * kWinThreads) return false;
-  const int bs = 4 * (16 - g.ns);
-  return g.bs[0] == bs && g.bs[1] == bs && g.tile[0] == bs + g.ns && g.tile[1] == bs + g.ns;
+// The window spread is the default where it applies; NUFFTX_SPREAD_WIN=0 selects the
+// LDS-atomic bank-spread kernel instead (measured on the 2D radial workload, 4M points,
+// 1024^2 grid: 142 us for the window kernel + 41 us for packing the points, against
+// 199-213 us).
+inline bool use_win_spread() {
+  const char* e = std::getenv("NUFFTX_SPREAD_WIN");
+  return e == nullptr || e[0] != '0';
 }
 
 }  // namespace nufftx

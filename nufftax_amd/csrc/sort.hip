@@ -250,6 +250,7 @@ struct RecArgs {
   int nf[2], bs[2], nb0, ns, sb;
   float4* recs;  // full records, or null
   uint2* crec;   // compact records, or null
+  int32_t* slot;  // inverse permutation (slot[i] = sorted position of point i), or null
 };
 
 __device__ __forceinline__ float4 point_record(const RecArgs& ra, int key, double x, double y, int i) {
@@ -360,6 +361,7 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
       const int pos = b0 + rank;
       if (valid && pos >= 0 && (int64_t)pos < M) {
         perm[pos] = (int32_t)i;
+        if (ra.slot) ra.slot[i] = pos;
         if constexpr (REC != 0) {
           const double x = (double)static_cast<const CT*>(ra.coord[0])[i];
           const double y = (double)static_cast<const CT*>(ra.coord[1])[i];
@@ -381,7 +383,8 @@ __global__ void __launch_bounds__(1024) k_scatter(const int32_t* __restrict__ ke
 // lowest lane of each key add its count the same way and hands the old offset to the others.
 template <typename CT>
 __global__ void __launch_bounds__(1024) k_scatter16(const int32_t* __restrict__ keys, int64_t M, int SC, int nbins,
-                                                    int key_bits, ScanArgs sa, int32_t* __restrict__ perm) {
+                                                    int key_bits, ScanArgs sa, int32_t* __restrict__ perm,
+                                                    int32_t* __restrict__ slot) {
   extern __shared__ uint32_t wc2[];  // [W][(nbins + 1) / 2]
   {
     const int n = min(*sa.nsub, sa.max_subprob);
@@ -450,7 +453,34 @@ __global__ void __launch_bounds__(1024) k_scatter16(const int32_t* __restrict__ 
       const int leader = m ? __ffsll((long long)m) - 1 : lane;
       const int off = __shfl((int)((old >> sh) & 0xFFFFu), leader, 64);
       const int pos = first[key] + off + rank;
-      if (valid && pos >= 0 && (int64_t)pos < M) perm[pos] = (int32_t)i;
+      if (valid && pos >= 0 && (int64_t)pos < M) {
+        perm[pos] = (int32_t)i;
+        if (slot) slot[i] = pos;
+      }
+    }
+  }
+}
+
+// Packed records of a 2D FP32 point set for the matrix-core window spread: point i of
+// transform t goes to packed[t][slot[i]] = {x word, y word, re c, im c} (compact words as
+// in compact_record, tile-relative to the point's bin from its stored key).  Coordinates,
+// keys, slots and strengths are read in index order; the 16-byte records are scattered.
+template <typename CT>
+__global__ void __launch_bounds__(256) k_pack2d(RecArgs ra, const int32_t* __restrict__ keys, int64_t M,
+                                                int64_t n_trans, const float2* __restrict__ c,
+                                                uint4* __restrict__ packed) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride) {
+    const int pos = ra.slot[i];
+    if ((unsigned)pos >= (unsigned)M) continue;
+    const double x = (double)static_cast<const CT*>(ra.coord[0])[i];
+    const double y = (double)static_cast<const CT*>(ra.coord[1])[i];
+    const uint2 w = compact_record(ra, keys[i], x, y);
+    const bool fin = isfinite(x) && isfinite(y);
+    for (int64_t t = 0; t < n_trans; ++t) {
+      float2 v = c[t * M + i];
+      if (!fin) v = make_float2(NAN, NAN);
+      packed[t * M + pos] = make_uint4(w.x, w.y, __float_as_uint(v.x), __float_as_uint(v.y));
     }
   }
 }
@@ -500,7 +530,7 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
     if (l16 > 65536)
       (void)hipFuncSetAttribute((const void*)k_scatter16<CT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l16);
     hipLaunchKernelGGL((k_scatter16<CT>), dim3(ps.NB), dim3(64 * W), l16, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
-                       ps.perm);
+                       ps.perm, ra.slot);
     return;
   }
   const size_t lds = (size_t)W * nbins * sizeof(int32_t);
@@ -523,7 +553,7 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
 }  // namespace
 
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
-                         int rec_kind, int sub_sb) {
+                         int rec_kind, int sub_sb, bool want_slot, int64_t n_trans) {
   hipError_t e;
   const int nbins = g.nbins;
   if ((int64_t)nbins * sizeof(int32_t) > 65536) {
@@ -575,8 +605,14 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   const bool crec = rec_kind == 2 && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
   if (recs && (e = grow_buffer(ps.recs, ps.cap_recs, M, ps.retired)) != hipSuccess) return e;
   if (crec && (e = grow_buffer(ps.crec, ps.cap_crec, M, ps.retired)) != hipSuccess) return e;
+  const bool slot = want_slot && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
+  if (slot && (e = grow_buffer(ps.slot, ps.cap_slot, M, ps.retired)) != hipSuccess) return e;
+  if (slot && n_trans > 0 &&
+      (e = grow_buffer(ps.packed, ps.cap_packed, M * n_trans, ps.retired)) != hipSuccess)
+    return e;
   ps.has_recs = false;
   ps.has_crec = false;
+  ps.has_slot = false;
   ps.recs_subbin = false;
   if ((e = grow_buffer(ps.keys, ps.cap_keys, M, ps.retired)) != hipSuccess) return e;
   if ((e = grow_buffer(ps.sub_start, ps.cap_sub_start, (int64_t)nbins + 1, ps.retired)) != hipSuccess) return e;
@@ -617,6 +653,7 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   ra.sb = sub_sb;
   ra.recs = recs ? ps.recs : nullptr;
   ra.crec = crec ? ps.crec : nullptr;
+  ra.slot = slot ? ps.slot : nullptr;
   if (ps.coord_f64) {
     if (g.dim == 1) launch_sort_passes<double, 1>(ka, ps, nbins, W, isc, c16, ra, s);
     if (g.dim == 2) launch_sort_passes<double, 2>(ka, ps, nbins, W, isc, c16, ra, s);
@@ -629,12 +666,40 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   e = hipGetLastError();
   if (e == hipSuccess && recs) ps.has_recs = true;
   if (e == hipSuccess && crec) ps.has_crec = true;
+  if (e == hipSuccess && slot) ps.has_slot = true;
   return e;
+}
+
+hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s) {
+  if (!ps.has_slot || g.dim != 2) return hipErrorInvalidValue;
+  if (ps.M == 0) return hipSuccess;
+  // sized by setpts for the plans that spread; a point set that is only interpolated
+  // (type 2) gets its buffer on its first spread (the reverse-mode derivative)
+  hipError_t e = grow_buffer(ps.packed, ps.cap_packed, ps.M * n_trans, ps.retired);
+  if (e != hipSuccess) return e;
+  RecArgs ra{};
+  ra.coord[0] = ps.coord[0];
+  ra.coord[1] = ps.coord[1];
+  ra.nf[0] = g.nf[0];
+  ra.nf[1] = g.nf[1];
+  ra.bs[0] = g.bs[0];
+  ra.bs[1] = g.bs[1];
+  ra.nb0 = g.nb[0];
+  ra.ns = g.ns;
+  ra.slot = ps.slot;
+  const unsigned nb = (unsigned)std::min<int64_t>((ps.M + 255) / 256, 8192);
+  if (ps.coord_f64)
+    hipLaunchKernelGGL(k_pack2d<double>, dim3(nb), dim3(256), 0, s, ra, ps.keys, ps.M, n_trans,
+                       static_cast<const float2*>(c), ps.packed);
+  else
+    hipLaunchKernelGGL(k_pack2d<float>, dim3(nb), dim3(256), 0, s, ra, ps.keys, ps.M, n_trans,
+                       static_cast<const float2*>(c), ps.packed);
+  return hipGetLastError();
 }
 
 void pointset_free(PointSet& ps) {
   void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.scan_sum, ps.bin_start, ps.sub_start, ps.subprob,
-                  ps.nsub, ps.recs, ps.crec, ps.sub_off, ps.cmax};
+                  ps.nsub, ps.recs, ps.crec, ps.sub_off, ps.cmax, ps.slot, ps.packed};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (void* p : ps.retired) (void)hipFree(p);
@@ -647,6 +712,10 @@ void pointset_free(PointSet& ps) {
   ps.cap_crec = 0;
   ps.sub_off = nullptr;
   ps.cmax = nullptr;
+  ps.slot = nullptr;
+  ps.packed = nullptr;
+  ps.has_slot = false;
+  ps.cap_slot = ps.cap_packed = 0;
   ps.has_recs = false;
   ps.recs_subbin = false;
   ps.cap_recs = ps.cap_sub_off = ps.cap_cmax = 0;

@@ -108,20 +108,23 @@ bool spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, h
   return false;
 }
 
-// The matrix-core window spread (2D FP32, ns <= 8, subproblems of <= 1024 points).
-template <typename CT>
+// The matrix-core window spread (2D FP32, ns <= 8, subproblems of <= 1024 points): the
+// points are packed in sorted order first (the sort left the inverse permutation), then
+// spread from those contiguous records.
 bool spread_win(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hipError_t& err) {
-  const PointSet& ps = *ga.ps;
-  if (!win2d_applicable(*ga.g, ga.prec, ps.maxsub) || !use_win_spread()) return false;
+  PointSet& ps = const_cast<PointSet&>(*ga.ps);
+  if (!win2d_geometry(*ga.g, ga.prec, ps.maxsub) || !ps.has_slot || !use_win_spread()) return false;
   SIArgs<float> a = make_si_args<float>(ga);
   auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;
-    const size_t lds = win_spread_lds<NS>(ps.maxsub);
-    if (lds + 1024 > (size_t)lds_device_max_bytes()) return false;
-    auto kern = a.crec ? k_spread_win2d<CT, NS, true> : k_spread_win2d<CT, NS, false>;
+    const bool exact = win_exact_tiles();
+    const size_t lds = exact ? win_spread_lds<NS, false>(ps.maxsub) : win_spread_lds<NS, true>(ps.maxsub);
+    if (lds + 2048 > (size_t)lds_device_max_bytes()) return false;
+    if ((err = pointset_pack2d(ps, *ga.g, c, ga.n_trans, s)) != hipSuccess) return true;
+    auto kern = exact ? k_spread_win2d<NS, false> : k_spread_win2d<NS, true>;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kWinThreads), lds, s, a,
-                       static_cast<const cplx<float>*>(c), static_cast<cplx<float>*>(fw));
+    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kWinThreads), lds, s, a, ps.maxsub,
+                       ps.packed, static_cast<cplx<float>*>(fw));
     err = hipGetLastError();
     return true;
   };
@@ -139,7 +142,7 @@ hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t
   if constexpr (std::is_same<T, float>::value) {
     hipError_t err = hipSuccess;
     if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma() && spread_mfma(ga, c, fw, s, err)) return err;
-    if (spread_win<CT>(ga, c, fw, s, err)) return err;
+    if (spread_win(ga, c, fw, s, err)) return err;
   }
   switch (ga.g->dim) {
     case 1: return spread_impl<T, CT, 1>(ga, c, fw, s);

@@ -8,13 +8,17 @@ the transform) and, where it applies, through adjointness.
 Bars (relative l2 on the subset):
   * 1D type 1, M = 100,000, 1024 modes, complex64, eps 1e-6: <= 2x the error of
     the complex64 restatement of the reference algorithm (oracle, full input) + 1e-7;
-  * 2D radial, M = 3,999,744, 512^2 modes, complex64, eps 1e-6: < 1e-5 (10 eps) for
-    type 1 and type 2; <T1 c, f> = <c, T2 f> to 1e-5 relative;
+  * 2D radial, M = 3,999,744, 512^2 modes, complex64, eps 1e-6: type 1 and type 2 each
+    <= 2x the error of the complex64 restatement of the reference algorithm run on the
+    full input (the reference-error proxy, same 64 modes / 64 points) + 1e-7, and < 1e-5;
+    <T1 c, f> = <c, T2 f> to 1e-5 relative;
   * 2D round trip, M = 1e6, 256^2 modes: complex64 gradients within 1e-4 of the
-    complex128 (eps 1e-12) gradients; complex128 coordinate gradients match
-    central differences (h = 1e-4) to 2e-3;
-  * 3D Gaussian clusters, M = 8e6, 128^3 modes, complex64: < 10 eps at eps 1e-4 and 1e-6;
-  * 3D complex128, M = 5e5, 64^3 modes, eps 1e-12: < 1e-10;
+    complex128 (eps 1e-12) gradients; complex128 coordinate gradients on 32 coordinates
+    match central differences at h = 1e-3 to 1e-2 (the O(h^2) truncation) and their
+    Richardson extrapolation with h / 2 to 2e-4;
+  * 3D Gaussian clusters, M = 8e6, 128^3 modes, complex64: type 1 < 10 eps at eps 1e-4
+    and 1e-6; type 2 on 64 seeded points <= 2x the complex64 proxy at those points + 1e-7;
+  * 3D complex128, M = 5e5, 64^3 modes, eps 1e-12: type 1 and type 2 < 1e-10;
   * 3D type 3, M = N = 2e5, complex128, eps 1e-9: < 1e-8.
 """
 
@@ -108,10 +112,17 @@ def test_2d_radial_4m_type1_type2_adjoint():
     modes = _rand_modes(rng, (512, 512))
     coords64 = [x32.astype(np.float64), y32.astype(np.float64)]
     truth1 = _direct_modes(coords64, c64.astype(np.complex128), modes, 1)
-    assert R.rel_l2(_pick(F.cpu().numpy(), modes, (512, 512)), truth1) < 1e-5
+    err1 = R.rel_l2(_pick(F.cpu().numpy(), modes, (512, 512)), truth1)
     idx = rng.integers(0, x.size, 64)
     truth2 = _direct_points(coords64, f64.astype(np.complex128), idx, -1)
-    assert R.rel_l2(C.cpu().numpy()[idx], truth2) < 1e-5
+    err2 = R.rel_l2(C.cpu().numpy()[idx], truth2)
+    # the reference-error proxy: the complex64 restatement on the full input
+    proxy1 = R.rel_l2(_pick(R.nufft_type1([x32, y32], c64, (512, 512), eps=1e-6, isign=1), modes, (512, 512)),
+                      truth1)
+    proxy2 = R.rel_l2(R.nufft_type2([x32[idx], y32[idx]], f64, eps=1e-6, isign=-1), truth2)
+    print(f"radial 4M: type 1 {err1:.2e} (proxy {proxy1:.2e}), type 2 {err2:.2e} (proxy {proxy2:.2e})")
+    assert err1 <= 2 * proxy1 + 1e-7 and err1 < 1e-5, (err1, proxy1)
+    assert err2 <= 2 * proxy2 + 1e-7 and err2 < 1e-5, (err2, proxy2)
     lhs = torch.vdot(F.reshape(-1).to(torch.complex128), fd.reshape(-1).to(torch.complex128))
     rhs = torch.vdot(cd.to(torch.complex128), C.to(torch.complex128))
     assert abs(complex(lhs - rhs)) <= 1e-5 * abs(complex(lhs))
@@ -146,14 +157,25 @@ def test_2d_roundtrip_1m_gradients():
             out = N.nufft2d1(xs, ys, N.nufft2d2(xs, ys, _t(f0), eps=1e-12), (256, 256), eps=1e-12)
             return float(torch.sum(torch.abs(out - _t(g)) ** 2) / 1e12)
 
-    gx = g64[1].cpu().numpy()
-    h = 1e-4
-    for j in rng.integers(0, x.size, 8):
-        xp, xm = x.copy(), x.copy()
-        xp[j] += h
-        xm[j] -= h
-        fd = (loss64(xp) - loss64(xm)) / (2 * h)
-        assert abs(fd - gx[j]) <= 2e-3 * abs(gx[j]) + 1e-9, (j, fd, gx[j])
+    gx = g64[1].cpu().numpy()[idx_fd := rng.choice(x.size, 32, replace=False)]
+
+    def central(h):
+        out = []
+        for j in idx_fd:
+            xp, xm = x.copy(), x.copy()
+            xp[j] += h
+            xm[j] -= h
+            out.append((loss64(xp) - loss64(xm)) / (2 * h))
+        return np.array(out)
+
+    # central differences at the configured step h = 1e-3 carry a truncation error of about
+    # (k h)^2 / 6 for modes up to |k| ~ 128 sqrt(2) (about 5e-3); Richardson extrapolation with
+    # h / 2 cancels that term, leaving the O(h^4) term and the FP64 rounding of the loss
+    d1, d2 = central(1e-3), central(5e-4)
+    rich = (4 * d2 - d1) / 3
+    print(f"FD rel l2: h=1e-3 {R.rel_l2(gx, d1):.2e}, Richardson {R.rel_l2(gx, rich):.2e}")
+    assert R.rel_l2(gx, d1) < 1e-2, R.rel_l2(gx, d1)
+    assert R.rel_l2(gx, rich) < 2e-4, R.rel_l2(gx, rich)
 
 
 @pytest.mark.parametrize("eps", [1e-4, 1e-6])
@@ -170,6 +192,24 @@ def test_3d_gaussian_clusters_8m_c64(eps):
     assert R.rel_l2(_pick(out, modes, (128, 128, 128)), truth) < 10 * eps
 
 
+@pytest.mark.parametrize("eps", [1e-4, 1e-6])
+def test_3d_gaussian_clusters_8m_c64_type2(eps):
+    import nufftax_amd as N
+
+    coords, _ = workloads.gaussian_mixture_3d(seed=0)
+    c32 = [x.astype(np.float32) for x in coords]
+    rng = np.random.default_rng(6)
+    f = (rng.standard_normal((128, 128, 128)) + 1j * rng.standard_normal((128, 128, 128))).astype(np.complex64)
+    out = N.nufft3d2(*[_t(x) for x in c32], _t(f), eps=eps).cpu().numpy()
+    idx = rng.integers(0, c32[0].size, 64)
+    truth = _direct_points([x.astype(np.float64) for x in c32], f.astype(np.complex128), idx, -1)
+    err = R.rel_l2(out[idx], truth)
+    # reference-error proxy at the same points: complex64 restatement (its interpolation is per point)
+    proxy = R.rel_l2(R.nufft_type2([x[idx] for x in c32], f, eps=eps, isign=-1), truth)
+    print(f"3D clusters 8M type 2 eps {eps}: {err:.2e} (proxy {proxy:.2e})")
+    assert err <= 2 * proxy + 1e-7, (err, proxy)
+
+
 def test_3d_c128_eps1e12_500k():
     import nufftax_amd as N
 
@@ -179,6 +219,11 @@ def test_3d_c128_eps1e12_500k():
     modes = _rand_modes(rng, (64, 64, 64))
     truth = _direct_modes(coords, c, modes, 1)
     assert R.rel_l2(_pick(out, modes, (64, 64, 64)), truth) < 1e-10
+    f = rng.standard_normal((64, 64, 64)) + 1j * rng.standard_normal((64, 64, 64))
+    out2 = N.nufft3d2(*[_t(x) for x in coords], _t(f), eps=1e-12).cpu().numpy()
+    idx = rng.integers(0, coords[0].size, 64)
+    truth2 = _direct_points(coords, f, idx, -1)
+    assert R.rel_l2(out2[idx], truth2) < 1e-10
 
 
 def test_3d_type3_200k_c128():

@@ -337,6 +337,42 @@ def test_batched_n_trans_matches_loop(rng):
             assert R.rel_l2(_np(out[i, j]), _np(single)) < 1e-6
 
 
+@pytest.mark.parametrize("dim,nm", [(1, (77,)), (2, (40, 36)), (3, (16, 12, 10))])
+def test_plain_transpose_identity_upsampfac_pair(rng, dim, nm):
+    """The reference's transpose rule: the linear transpose of type 1 (isign s, upsampfac
+    sigma) is type 2 with the same isign and the backward upsampfac (primitives.py:191-194,
+    235-241; tests/test_linear_solve.py:23-46), so sum_k f_k T1(c)_k = sum_j c_j T2(f)_j
+    (bilinear, no conjugation) with T1 at upsampfac 2.0 and T2 at 1.25, the type-2 plan
+    made for isign -1 and run through nufftx_execute_sign with +1."""
+    from nufftax_amd._plan import F32, transform_plan
+
+    M = 20000
+    xs = [_t(rng.uniform(-np.pi, np.pi, M), torch.float32) for _ in range(dim)]
+    c = _t(rng.standard_normal(M) + 1j * rng.standard_normal(M), torch.complex64)
+    f = _t(rng.standard_normal(tuple(reversed(nm))) + 1j * rng.standard_normal(tuple(reversed(nm))),
+           torch.complex64)
+    p1 = transform_plan(1, dim, nm, 1, 1e-6, 1, 2.0, F32, DEV)
+    p2 = transform_plan(2, dim, nm, 1, 1e-6, -1, 1.25, F32, DEV)
+    assert p2.grid_size(0) != p1.grid_size(0)  # the two plans really use different grids
+    F = torch.empty(tuple(reversed(nm)), dtype=torch.complex64, device=DEV)
+    C = torch.empty(M, dtype=torch.complex64, device=DEV)
+    with p1.lock:
+        p1.setpts(xs)
+        p1.execute(c, F, 1)
+    with p2.lock:
+        p2.setpts(xs)
+        p2.execute(C, f, 1)
+    lhs = torch.sum(f.to(torch.complex128) * F.to(torch.complex128))
+    rhs = torch.sum(c.to(torch.complex128) * C.to(torch.complex128))
+    scale = float(torch.linalg.vector_norm(f.to(torch.complex128)) * torch.linalg.vector_norm(F.to(torch.complex128)))
+    assert abs(complex(lhs - rhs)) <= 1e-5 * scale, (complex(lhs), complex(rhs))
+    # and each side equals the exact bilinear form (FP64 direct sums of both transforms)
+    xn = [_np(x).astype(np.float64) for x in xs]
+    T1 = R.direct_type1(xn, _np(c).astype(np.complex128), nm, isign=1)
+    exact = np.sum(_np(f).astype(np.complex128) * T1)
+    assert abs(complex(lhs) - exact) <= 1e-5 * scale
+
+
 def test_type1_type2_adjoint(rng):
     import nufftax_amd as N
 
