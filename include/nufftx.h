@@ -189,6 +189,29 @@ int nufftx_get_sort_info(nufftx_plan plan, int inner, int* nbins, int* bin_size 
    (nbins + 1 int32) into device buffers, on the plan stream.  Either may be NULL. */
 int nufftx_copy_sort(nufftx_plan plan, int inner, int32_t* perm, int32_t* bin_start);
 
+/* ---- spreading without device allocation ---------------------------------------- */
+
+/* Working memory of nufftx_spread_noalloc: a fixed device array of this many bytes per GPU,
+   held from the library's first use on that GPU until the process ends. */
+#define NUFFTX_NOALLOC_SCRATCH_BYTES (16 << 20)
+/* Largest M nufftx_spread_noalloc accepts. */
+#define NUFFTX_NOALLOC_MAX_POINTS (1 << 18)
+
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the result of
+ * nufftx_makeplan_grid + nufftx_setpts + nufftx_spread for one transform (n_trans = 1),
+ *   fw[nf3][nf2][nf1] = spread(c[M])   (spread.py:285-339, 435-494, 596-658),
+ * with every working buffer (sort state, packed points, strength bounds) in a fixed device array
+ * of NUFFTX_NOALLOC_SCRATCH_BYTES instead of memory allocated per call; no hipMalloc, hipFree or
+ * allocating library call.  Inputs and output are the caller's device buffers; fw is overwritten.
+ * Enqueued on stream.  Limits: M <= NUFFTX_NOALLOC_MAX_POINTS, and the sort state of the grid
+ * must fit the array (NUFFTX_ERR_INVALID_ARG otherwise).  Calls that overlap on one GPU share
+ * the array and race: serialise them (one stream, or events).
+ */
+int nufftx_spread_noalloc(int dim, const int64_t* nf, int nspread, double beta, double c, int precision, int64_t M,
+                          const void* x, const void* y, const void* z, const void* cstr, void* fw, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,7 +65,13 @@ SIGNATURES = {
                                                     _vp]),
     "nufftx_deconvolve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int64_p, _c_int64_p, ctypes.c_int64, _vp, _vp, _vp,
                                          ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+    "nufftx_spread_noalloc": (ctypes.c_int, [ctypes.c_int, _c_int64_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
+
+# nufftx_spread_noalloc limits (include/nufftx.h)
+NOALLOC_SCRATCH_BYTES = 16 << 20
+NOALLOC_MAX_POINTS = 1 << 18
 
 _lib = None
 _lock = threading.Lock()

@@ -13,6 +13,9 @@ namespace nufftx {
 
 enum Prec { kF32 = 0, kF64 = 1 };
 
+// Records msg as the calling thread's last error (nufftx_last_error) and returns code.
+int fail(int code, const std::string& msg);
+
 // Binning of one fine grid: bins of bs[d] cells, tiles of bs[d] + ns cells.
 struct Geometry {
   int dim = 1;
@@ -95,6 +98,16 @@ int lds_device_max_bytes();
 // rec_kind (2D only): also write point records, 1 = full (matrix-core kernels),
 // 2 = compact (LDS-tiled kernels); sub_sb: sub-bin side of the matrix-core
 // geometry (16 - ns) to record sub-bins in full records, 0 otherwise
+// Sizes of a point set's sort state (pointset_sort): waves per chunk, points per wave, 16-bit
+// counters, chunks (histogram rows), histogram entries, segment partials, subproblem capacity.
+struct SortSizes {
+  int W = 1;
+  int64_t SC = 64;
+  bool c16 = false;
+  int64_t NB = 1, nhist = 0, nparts = 0, max_sub = 0;
+};
+SortSizes sort_sizes(int64_t M, const Geometry& g, const SortTuning& tune, int rec_kind);
+
 // want_slot (2D): also write the inverse permutation, and size the packed records for n_trans
 // transforms (n_trans 0: left to the first spread)
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,

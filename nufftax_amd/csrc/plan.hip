@@ -19,12 +19,12 @@ namespace nufftx {
 // ------------------------------------------------------------------ errors
 namespace {
 thread_local std::string g_last_error;
+}  // namespace
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
-}  // namespace
 
 #define HIPCHK(expr)                                                                              \
   do {                                                                                            \

@@ -552,20 +552,9 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
 
 }  // namespace
 
-hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
-                         int rec_kind, int sub_sb, bool want_slot, int64_t n_trans) {
-  hipError_t e;
+SortSizes sort_sizes(int64_t M, const Geometry& g, const SortTuning& tune, int rec_kind) {
+  SortSizes z;
   const int nbins = g.nbins;
-  if ((int64_t)nbins * sizeof(int32_t) > 65536) {
-    err = "bin count exceeds the LDS histogram capacity (16384 bins)";
-    return hipErrorInvalidValue;
-  }
-  if (ps.M > (int64_t)INT32_MAX - 1) {
-    err = "number of points exceeds 2^31 - 2";
-    return hipErrorInvalidValue;
-  }
-  ps.maxsub = tune.maxsub;
-  const int64_t M = ps.M;
   // waves per chunk (up to 16): their LDS counters (W * nbins ints) must fit the device's LDS
   const int64_t lds_max = lds_device_max_bytes();
   int W = 16;
@@ -593,13 +582,40 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
       c16 = true;
     }
   }
-  const int64_t NB = std::max<int64_t>(1, (M + W * SC - 1) / (W * SC));
+  z.W = W;
+  z.SC = SC;
+  z.c16 = c16;
+  z.NB = std::max<int64_t>(1, (M + W * SC - 1) / (W * SC));
+  z.nhist = (int64_t)nbins * z.NB;
+  z.nparts = ((z.NB + kRowSeg - 1) / kRowSeg) * nbins;
+  z.max_sub = nbins + (M + tune.maxsub - 1) / tune.maxsub;
+  return z;
+}
+
+hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
+                         int rec_kind, int sub_sb, bool want_slot, int64_t n_trans) {
+  hipError_t e;
+  const int nbins = g.nbins;
+  if ((int64_t)nbins * sizeof(int32_t) > 65536) {
+    err = "bin count exceeds the LDS histogram capacity (16384 bins)";
+    return hipErrorInvalidValue;
+  }
+  if (ps.M > (int64_t)INT32_MAX - 1) {
+    err = "number of points exceeds 2^31 - 2";
+    return hipErrorInvalidValue;
+  }
+  ps.maxsub = tune.maxsub;
+  const int64_t M = ps.M;
+  const SortSizes z = sort_sizes(M, g, tune, rec_kind);
+  const int W = z.W;
+  const int64_t SC = z.SC;
+  const bool c16 = z.c16;
+  const int64_t NB = z.NB;
   ps.NB = (int)NB;
   ps.CH = (int)(W * SC);
-  const int64_t nhist = (int64_t)nbins * NB;
-  const int64_t nseg = (NB + kRowSeg - 1) / kRowSeg;
-  const int64_t nparts = nseg * nbins;
-  const int64_t max_sub = nbins + (M + ps.maxsub - 1) / ps.maxsub;
+  const int64_t nhist = z.nhist;
+  const int64_t nparts = z.nparts;
+  const int64_t max_sub = z.max_sub;
   if ((e = grow_buffer(ps.perm, ps.cap_M, M, ps.retired)) != hipSuccess) return e;
   const bool recs = rec_kind == 1 && g.dim == 2 && M > 0;
   const bool crec = rec_kind == 2 && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
