@@ -419,7 +419,9 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
     """Replay reqs with Poisson arrivals at qps (qps <= 0: every request arrives at once, which
     measures the saturated throughput).  mode "server": NufftServer (max_units point sets per
     shared launch; 1 turns cross-point batching off); mode "per-call": the same transforms called
-    one at a time (each synchronised) in arrival order."""
+    one at a time (each synchronised) in arrival order; mode "per-call-async": the same calls
+    issued back to back on one stream without synchronising, each followed by an event whose
+    completion (seen by a waiting thread, in issue order) is the request's completion time."""
     rng = np.random.default_rng(seed + 1)
     if qps > 0:
         arrivals = np.cumsum(rng.exponential(1.0 / qps, len(reqs)))
@@ -458,6 +460,37 @@ def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams:
                 f.result()
             records = [f.request for f in futs]
             built = srv.plans_built - built0
+    elif mode == "per-call-async":
+        done_q: "queue.Queue" = queue.Queue()
+
+        def complete():
+            while True:
+                item = done_q.get()
+                if item is None:
+                    break
+                ev, r, out = item
+                ev.synchronize()
+                r.t_done = time.perf_counter()
+                del out
+
+        waiter = threading.Thread(target=complete, daemon=True)
+        waiter.start()
+        t0 = time.perf_counter()
+        for (kind, coords, data, nm), ta in zip(reqs, arrivals):
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            r = ServeRequest(kind, coords, data, nm, 1e-6, 1 if kind == 1 else -1)
+            r.t_submit = t0 + ta
+            r.t_start = time.perf_counter()
+            out = _direct_call(kind, coords, data, nm)
+            ev = torch.cuda.Event()
+            ev.record(torch.cuda.current_stream(dev))
+            r.t_issued = time.perf_counter()
+            done_q.put((ev, r, out))
+            records.append(r)
+        done_q.put(None)
+        waiter.join()
     else:
         t0 = time.perf_counter()
         for (kind, coords, data, nm), ta in zip(reqs, arrivals):
@@ -529,7 +562,8 @@ def main(argv=None):
     for qps in args.qps:
         res = {"qps": qps, "streams": args.streams,
                "server": run_stream(reqs, qps, args.seed, "server", args.streams, args.max_units),
-               "per_call": run_stream(reqs, qps, args.seed, "per-call")}
+               "per_call": run_stream(reqs, qps, args.seed, "per-call"),
+               "per_call_async": run_stream(reqs, qps, args.seed, "per-call-async")}
         if args.compare_batched:
             res[f"server_units{args.compare_batched}"] = run_stream(reqs, qps, args.seed, "server", args.streams,
                                                                    args.compare_batched)
