@@ -108,8 +108,9 @@ int nufftx_execute_sign(nufftx_plan plan, void* c, void* f, int isign);
  * No counterpart in the reference (its nearest relative is the n_trans batching of
  * nufftax/core/spread.py:100-173); for serving loops that batch requests of uneven M and
  * grid size.  Up to 8 FP32 2D/3D plans with the same kernel width, tile shape, n_trans and
- * stream share one spreading (type 1) or interpolation (type 2) launch; other batches run plan
- * by plan.
+ * stream share one spreading (type 1, the LDS-atomic kernel) or interpolation (type 2) launch;
+ * other batches run plan by plan.  Shared launches equal the plans run one by one to FP32
+ * rounding (relative l2 within 1e-6), not bit for bit.
  * The plans must be distinct.
  */
 int nufftx_execute_many(int count, const nufftx_plan* plans, void* const* c, void* const* f);
@@ -129,7 +130,48 @@ int nufftx_execute_many(int count, const nufftx_plan* plans, void* const* c, voi
 int nufftx_vjp(nufftx_plan plan, const void* src, const void* g, void* grad_src, void* grad_x, void* grad_y,
                void* grad_z);
 
+/* Frees a plan made by nufftx_makeplan / nufftx_makeplan_grid (not one from the plan cache). */
 int nufftx_destroy(nufftx_plan plan);
+
+/* ---- plan cache (for call sites that plan per call, such as an FFI handler) ---- */
+
+/*
+ * The reference plans nothing across calls: each primitive impl recomputes kernel parameters,
+ * grid sizes and phihat (nufftax/transforms/primitives.py:64-70 lowers every call through
+ * mlir.lower_fun(impl)).  A handler that replaces that lowering calls nufftx_plan_acquire with
+ * the primitive's parameters and its stream, then nufftx_setpts + nufftx_execute(_sign), then
+ * nufftx_plan_release.  Acquire returns an idle cached plan of the same parameters, stream and
+ * device, or makes one (nufftx_makeplan's arguments and errors); the caller holds it alone until
+ * release, so concurrent callers never share a plan.  Type-1/2 plans are shared between the
+ * two signs: acquire sets the plan's sign to isign.  Released plans keep their buffers: a
+ * setpts + execute of the same sizes on a reused plan allocates no device memory and does not
+ * synchronise the host.  Release does not wait for the GPU (work in flight on a released plan
+ * is ordered before the next holder's by the stream).  At most 32 idle plans are kept; beyond
+ * that release destroys the least recently released one after synchronising its device.
+ * Thread-safe.  A cached plan must not be passed to nufftx_destroy or nufftx_set_stream.  A
+ * stream that cached plans were acquired for may be destroyed once they are released (an idle
+ * plan is only handed out again for a stream with the same handle).
+ */
+int nufftx_plan_acquire(int type, int dim, const int64_t* n_modes, int isign, int64_t n_trans, double eps,
+                        double upsampfac, int precision, int modeord, void* stream, nufftx_plan* out);
+
+/* nufftx_plan_acquire for grid-operator plans (nufftx_makeplan_grid's arguments). */
+int nufftx_plan_acquire_grid(int dim, const int64_t* nf, int nspread, double beta, double kc, int64_t n_trans,
+                             int precision, void* stream, nufftx_plan* out);
+
+/* Return an acquired plan to the cache. */
+int nufftx_plan_release(nufftx_plan plan);
+
+/* Destroy every idle cached plan (held plans stay), after synchronising their devices;
+   returns how many were destroyed. */
+int nufftx_plan_cache_clear(void);
+
+/* Idle and held plans in the cache, plans made by acquire and acquires served from the cache. */
+int nufftx_plan_cache_stats(int64_t* idle, int64_t* held, int64_t* made, int64_t* reused);
+
+/* Device allocations (hipMalloc calls) the library has made so far in this process, rocFFT's
+   own plan-time buffers aside: steady-state setpts/execute leave it unchanged. */
+int64_t nufftx_device_allocations(void);
 
 /* ---- stage-level operators (nufftax.core) ------------------------------- */
 

@@ -65,6 +65,20 @@ SIGNATURES = {
                                                     _vp]),
     "nufftx_deconvolve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_int64_p, _c_int64_p, ctypes.c_int64, _vp, _vp, _vp,
                                          ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+    "nufftx_plan_acquire": (
+        ctypes.c_int,
+        [ctypes.c_int, ctypes.c_int, _c_int64_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+         ctypes.c_int, ctypes.c_int, _vp, ctypes.POINTER(_vp)],
+    ),
+    "nufftx_plan_acquire_grid": (
+        ctypes.c_int,
+        [ctypes.c_int, _c_int64_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_int, _vp,
+         ctypes.POINTER(_vp)],
+    ),
+    "nufftx_plan_release": (ctypes.c_int, [_vp]),
+    "nufftx_plan_cache_clear": (ctypes.c_int, []),
+    "nufftx_plan_cache_stats": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int64)] * 4),
+    "nufftx_device_allocations": (ctypes.c_int64, []),
     "nufftx_spread_noalloc": (ctypes.c_int, [ctypes.c_int, _c_int64_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp]),
 }

@@ -335,17 +335,18 @@ bool fft2_applicable(const DeconvArgs& a) {
   return true;
 }
 
-hipError_t fft2_plan_init(Fft2Plan& fp, const int* nf) {
+hipError_t fft2_plan_init(Fft2Plan& fp, const DeconvArgs& a) {
+  hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * (int64_t)a.nm[0] * a.nf[1], fp.retired);
+  if (e != hipSuccess) return e;
   for (int d = 0; d < 2; ++d) {
-    const int n = nf[d];
+    const int n = a.nf[d];
     std::vector<float2> h((size_t)n / 2);
     for (int k = 0; k < n / 2; ++k) {
       const double ang = -2.0 * 3.14159265358979323846 * (double)k / (double)n;
       h[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
     float2* p = nullptr;
-    hipError_t e = hipMalloc((void**)&p, h.size() * sizeof(float2));
-    if (e != hipSuccess) return e;
+    if ((e = dev_malloc((void**)&p, h.size() * sizeof(float2))) != hipSuccess) return e;
     if ((e = hipMemcpy(p, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) {
       (void)hipFree(p);
       return e;

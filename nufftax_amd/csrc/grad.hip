@@ -26,8 +26,8 @@ hipError_t grad_impl(const GridOpArgs& ga, const void* G, const void* P, void* g
       if constexpr (std::is_same<T, float>::value && D >= 2 && NS <= 8) {
         // 16-byte tile reads, as in the value interpolation (NUFFTX_INTERP_WIDE=0: 8-byte)
         const char* ev = std::getenv("NUFFTX_INTERP_WIDE");
-        if ((a.tile[0] % 2) == 0 && !(ev && std::atoi(ev) == 0)) {
-          const size_t lw = lds + 2 * sizeof(cplx<float>);
+        const size_t lw = lds + 2 * sizeof(cplx<float>);  // the tile plus its two zero cells
+        if ((a.tile[0] % 2) == 0 && lw <= (size_t)lds_device_max_bytes() && !(ev && std::atoi(ev) == 0)) {
           auto kw = k_gather_wide_deriv<CT, D, NS>;
           if (lw > 65536) (void)hipFuncSetAttribute((const void*)kw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw);
           hipLaunchKernelGGL(kw, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lw, s, a,

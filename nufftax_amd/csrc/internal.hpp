@@ -29,6 +29,9 @@ struct Geometry {
   int64_t grid_cells() const { return (int64_t)nf[0] * nf[1] * nf[2]; }
 };
 
+// per-block partial maxima of the strength bound (k_absmax_part): at most this many per transform
+constexpr int kAbsmaxBlocks = 1024;
+
 // A sorted set of points on one geometry.  Coordinates are not copied: the
 // kernels re-read them through perm and refold them in FP64.
 struct PointSet {
@@ -60,7 +63,7 @@ struct PointSet {
   uint2* crec = nullptr;         // M compact records (LDS-tiled kernels)
   int64_t cap_crec = 0;
   int32_t* sub_off = nullptr;    // max_subprob * 17 sub-bin starts per subproblem
-  float* cmax = nullptr;          // n_trans: max |re|, |im| of the strengths (matrix-core spread)
+  float* cmax = nullptr;          // n_trans: max |re|, |im| of the strengths, then the per-block partials
   // 2D FP32 point sets spread by the matrix-core window kernel: the inverse permutation
   // (slot[i] = sorted position of point i, written by the sort) and, per execute, every
   // transform's points packed in sorted order as {x word, y word, re c, im c} (compact
@@ -122,13 +125,17 @@ hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64
 // `retired` and freed when its owner is destroyed; capacity grows by at least half
 // each time, so the retired buffers stay below twice the live one.  On failure the old
 // buffer and capacity are kept.
+// Every device allocation of the library goes through dev_malloc, which counts them
+// (nufftx_device_allocations reports the count).
+hipError_t dev_malloc(void** p, size_t bytes);
+
 template <typename T>
 hipError_t grow_buffer(T*& p, int64_t& cap, int64_t need, std::vector<void*>& retired) {
   if (need <= cap && p) return hipSuccess;
   int64_t n = need > 1 ? need : 1;
   if (p && cap + cap / 2 > n) n = cap + cap / 2;
   T* q = nullptr;
-  hipError_t e = hipMalloc((void**)&q, (size_t)n * sizeof(T));
+  hipError_t e = dev_malloc((void**)&q, (size_t)n * sizeof(T));
   if (e != hipSuccess) return e;
   if (p) retired.push_back(p);
   p = q;
@@ -190,7 +197,8 @@ struct Fft2Plan {
   std::vector<void*> retired;
 };
 bool fft2_applicable(const DeconvArgs& a);
-hipError_t fft2_plan_init(Fft2Plan& fp, const int* nf);
+// twiddle tables and the intermediate buffer for a.n_trans transforms (no allocation at run time)
+hipError_t fft2_plan_init(Fft2Plan& fp, const DeconvArgs& a);
 void fft2_plan_free(Fft2Plan& fp);
 // grid (FFT of the spread grid) -> deconvolved modes, and modes -> padded, deconvolved,
 // transformed grid; isign selects the exponent sign (both unnormalised)

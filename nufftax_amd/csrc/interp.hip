@@ -25,8 +25,8 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
         // 16-byte tile reads (k_gather_wide) unless NUFFTX_INTERP_WIDE=0; rows of an even
         // number of cells keep the reads aligned
         const char* ev = std::getenv("NUFFTX_INTERP_WIDE");
-        if ((a.tile[0] % 2) == 0 && !(ev && std::atoi(ev) == 0)) {
-          const size_t lw = lds + 2 * sizeof(cplx<float>);
+        const size_t lw = lds + 2 * sizeof(cplx<float>);  // the tile plus its two zero cells
+        if ((a.tile[0] % 2) == 0 && lw <= (size_t)lds_device_max_bytes() && !(ev && std::atoi(ev) == 0)) {
           auto kw = k_gather_wide<CT, D, NS, false>;
           if constexpr (D == 2)
             if (a.crec) kw = k_gather_wide<CT, D, NS, true>;

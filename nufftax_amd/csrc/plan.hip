@@ -2,6 +2,7 @@
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -24,6 +25,16 @@ thread_local std::string g_last_error;
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
+}
+
+// ------------------------------------------------------------------ device allocations
+namespace {
+std::atomic<int64_t> g_dev_allocs{0};
+}  // namespace
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+  g_dev_allocs.fetch_add(1, std::memory_order_relaxed);
+  return hipMalloc(p, bytes);
 }
 
 #define HIPCHK(expr)                                                                              \
@@ -228,7 +239,7 @@ int fft_make(FftPlan& p, int dim, const int nf[3], int64_t n_trans, int prec) {
     return fail(NUFFTX_ERR_FFT, "rocfft_execution_info_create failed");
   const size_t w = std::max(w1, w2);
   if (w > 0) {
-    HIPCHK(hipMalloc(&p.work, w));
+    HIPCHK(dev_malloc(&p.work, w));
     if (rocfft_execution_info_set_work_buffer(p.info, p.work, w) != rocfft_status_success)
       return fail(NUFFTX_ERR_FFT, "rocfft_execution_info_set_work_buffer failed");
   }
@@ -248,7 +259,7 @@ int fft_exec(FftPlan& p, void* data, int isign, hipStream_t s) {
 template <typename T>
 int dev_alloc(T*& p, size_t n) {
   if (p) return NUFFTX_OK;
-  if (hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+  if (dev_malloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
     p = nullptr;
     return fail(NUFFTX_ERR_ALLOC, "hipMalloc failed (" + std::to_string(n * sizeof(T)) + " bytes)");
   }
@@ -303,6 +314,7 @@ struct nufftx_plan_s {
   PointSet ps, pt;
   SortTuning tune;
   bool pts_set = false;
+  bool cached = false;  // owned by the plan cache (nufftx_plan_acquire): released, never destroyed
   void* grid = nullptr;
   void* grid_in = nullptr;
   void* fac[3] = {nullptr, nullptr, nullptr};
@@ -343,6 +355,33 @@ void plan_free(nufftx_plan p) {
   p->retired.clear();
 }
 
+// nufftx_makeplan's argument checks (no device call)
+int check_plan_args(int type, int dim, const int64_t* n_modes, int64_t n_trans, double eps, double upsampfac,
+                    int precision, int modeord) {
+  if (type < 1 || type > 3) return fail(NUFFTX_ERR_INVALID_ARG, "type must be 1, 2 or 3");
+  if (dim < 1 || dim > 3) return fail(NUFFTX_ERR_INVALID_ARG, "dim must be 1, 2 or 3");
+  if (!n_modes) return fail(NUFFTX_ERR_INVALID_ARG, "n_modes is null");
+  if (n_trans < 1 || n_trans > 65535) return fail(NUFFTX_ERR_INVALID_ARG, "n_trans must be in [1, 65535]");
+  if (!(eps > 0)) return fail(NUFFTX_ERR_INVALID_ARG, "eps must be positive");
+  if (!(upsampfac > 1.0) || upsampfac > 16.0) return fail(NUFFTX_ERR_INVALID_ARG, "upsampfac must be in (1, 16]");
+  if (precision != NUFFTX_F32 && precision != NUFFTX_F64) return fail(NUFFTX_ERR_INVALID_ARG, "bad precision");
+  if (modeord != 0 && modeord != 1) return fail(NUFFTX_ERR_INVALID_ARG, "modeord must be 0 or 1");
+  for (int d = 0; d < dim; ++d)
+    if (n_modes[d] < 1 || n_modes[d] > (1LL << 28)) return fail(NUFFTX_ERR_INVALID_ARG, "n_modes out of range");
+  return NUFFTX_OK;
+}
+
+// nufftx_makeplan_grid's argument checks (no device call)
+int check_grid_args(int dim, const int64_t* nf, int nspread, int64_t n_trans, int precision) {
+  if (dim < 1 || dim > 3 || !nf) return fail(NUFFTX_ERR_INVALID_ARG, "dim must be 1, 2 or 3");
+  if (nspread < 2 || nspread > 16 || nspread % 2) return fail(NUFFTX_ERR_INVALID_ARG, "nspread must be even in [2,16]");
+  if (n_trans < 1 || n_trans > 65535) return fail(NUFFTX_ERR_INVALID_ARG, "n_trans must be in [1, 65535]");
+  if (precision != NUFFTX_F32 && precision != NUFFTX_F64) return fail(NUFFTX_ERR_INVALID_ARG, "bad precision");
+  for (int d = 0; d < dim; ++d)
+    if (nf[d] < 1 || nf[d] > (1LL << 30)) return fail(NUFFTX_ERR_INVALID_ARG, "nf out of range");
+  return NUFFTX_OK;
+}
+
 int check_plan(nufftx_plan p) {
   if (!p) return fail(NUFFTX_ERR_INVALID_ARG, "null plan");
   return NUFFTX_OK;
@@ -367,6 +406,13 @@ int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g, bool interp, boo
   if (subsort && ps.has_recs) {
     e = pointset_subsort(ps, g, p->stream);
     if (e != hipSuccess) return fail(NUFFTX_ERR_HIP, std::string("sub-bin sort: ") + hipGetErrorString(e));
+  }
+  // the FP32 spreading kernels' strength bound (n_trans values and the per-block partial
+  // maxima) is sized here, so that execute allocates nothing for a point set of this size
+  if (p->prec == kF32) {
+    const int64_t nb = std::min<int64_t>((ps.M + 1023) / 1024, kAbsmaxBlocks);
+    if (grow_buffer(ps.cmax, ps.cap_cmax, p->n_trans * (1 + std::max<int64_t>(nb, 1)), ps.retired) != hipSuccess)
+      return fail(NUFFTX_ERR_ALLOC, "strength bound buffer");
   }
   return NUFFTX_OK;
 }
@@ -478,16 +524,7 @@ int nufftx_makeplan(int type, int dim, const int64_t* n_modes, int isign, int64_
                     double upsampfac, int precision, int modeord, void* stream, nufftx_plan* out) {
   if (!out) return fail(NUFFTX_ERR_INVALID_ARG, "plan output pointer is null");
   *out = nullptr;
-  if (type < 1 || type > 3) return fail(NUFFTX_ERR_INVALID_ARG, "type must be 1, 2 or 3");
-  if (dim < 1 || dim > 3) return fail(NUFFTX_ERR_INVALID_ARG, "dim must be 1, 2 or 3");
-  if (!n_modes) return fail(NUFFTX_ERR_INVALID_ARG, "n_modes is null");
-  if (n_trans < 1 || n_trans > 65535) return fail(NUFFTX_ERR_INVALID_ARG, "n_trans must be in [1, 65535]");
-  if (!(eps > 0)) return fail(NUFFTX_ERR_INVALID_ARG, "eps must be positive");
-  if (!(upsampfac > 1.0) || upsampfac > 16.0) return fail(NUFFTX_ERR_INVALID_ARG, "upsampfac must be in (1, 16]");
-  if (precision != NUFFTX_F32 && precision != NUFFTX_F64) return fail(NUFFTX_ERR_INVALID_ARG, "bad precision");
-  if (modeord != 0 && modeord != 1) return fail(NUFFTX_ERR_INVALID_ARG, "modeord must be 0 or 1");
-  for (int d = 0; d < dim; ++d)
-    if (n_modes[d] < 1 || n_modes[d] > (1LL << 28)) return fail(NUFFTX_ERR_INVALID_ARG, "n_modes out of range");
+  if (int r = check_plan_args(type, dim, n_modes, n_trans, eps, upsampfac, precision, modeord)) return r;
 
   nufftx_plan p = new nufftx_plan_s();
   p->type = type;
@@ -544,8 +581,8 @@ int nufftx_makeplan(int type, int dim, const int64_t* n_modes, int isign, int64_
     // shuffle kernel.  So the pad direction is fused by default; NUFFTX_FUSED_FFT fuses both
     // directions, NUFFTX_ROCFFT neither.
     if (fft2_applicable(dargs(p, false)) && !std::getenv("NUFFTX_ROCFFT")) {
-      if (fft2_plan_init(p->fft2, p->g.nf) != hipSuccess) {
-        r = fail(NUFFTX_ERR_ALLOC, "fused FFT twiddle tables");
+      if (fft2_plan_init(p->fft2, dargs(p, false)) != hipSuccess) {
+        r = fail(NUFFTX_ERR_ALLOC, "fused FFT twiddle tables and intermediate buffer");
         goto fail_out;
       }
       p->fft2_pad = true;
@@ -576,12 +613,7 @@ int nufftx_makeplan_grid(int dim, const int64_t* nf, int nspread, double beta, d
                          int precision, void* stream, nufftx_plan* out) {
   if (!out) return fail(NUFFTX_ERR_INVALID_ARG, "plan output pointer is null");
   *out = nullptr;
-  if (dim < 1 || dim > 3 || !nf) return fail(NUFFTX_ERR_INVALID_ARG, "dim must be 1, 2 or 3");
-  if (nspread < 2 || nspread > 16 || nspread % 2) return fail(NUFFTX_ERR_INVALID_ARG, "nspread must be even in [2,16]");
-  if (n_trans < 1 || n_trans > 65535) return fail(NUFFTX_ERR_INVALID_ARG, "n_trans must be in [1, 65535]");
-  if (precision != NUFFTX_F32 && precision != NUFFTX_F64) return fail(NUFFTX_ERR_INVALID_ARG, "bad precision");
-  for (int d = 0; d < dim; ++d)
-    if (nf[d] < 1 || nf[d] > (1LL << 30)) return fail(NUFFTX_ERR_INVALID_ARG, "nf out of range");
+  if (int r = check_grid_args(dim, nf, nspread, n_trans, precision)) return r;
   nufftx_plan p = new nufftx_plan_s();
   p->type = 0;
   p->dim = dim;
@@ -607,6 +639,7 @@ int64_t nufftx_plan_grid_size(nufftx_plan p, int d, int inner) {
 
 int nufftx_set_stream(nufftx_plan p, void* stream) {
   if (int r = check_plan(p)) return r;
+  if (p->cached) return fail(NUFFTX_ERR_STATE, "a cached plan keeps the stream it was acquired for");
   p->stream = static_cast<hipStream_t>(stream);
   return NUFFTX_OK;
 }
@@ -768,6 +801,7 @@ int nufftx_vjp(nufftx_plan p, const void* src, const void* g, void* grad_src, vo
 
 int nufftx_destroy(nufftx_plan p) {
   if (!p) return NUFFTX_OK;
+  if (p->cached) return fail(NUFFTX_ERR_STATE, "a cached plan is returned with nufftx_plan_release");
   plan_free(p);
   delete p;
   return NUFFTX_OK;
@@ -880,5 +914,224 @@ int nufftx_deconvolve(int direction, int dim, const int64_t* n_modes, const int6
     LAUNCH(launch_deconv_pad(a, in, out, s));
   return NUFFTX_OK;
 }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ plan cache
+// Plans keyed by their parameters, the stream and the device.  A plan is held by one caller
+// between acquire and release; release returns it to the idle pool, where the next acquire
+// with the same key takes it.  Work that a released plan still has in flight is ordered
+// before the next holder's by the key's stream, so release does not wait for the GPU.  At
+// most kCacheIdleMax idle plans are kept: beyond that the least recently released one is
+// destroyed (after a synchronisation of its device, as its buffers are freed).
+namespace {
+
+struct CacheKey {
+  int kind = 0;  // 0 transform (nufftx_makeplan), 1 grid operator (nufftx_makeplan_grid)
+  int type = 0, dim = 0, isign = 0, prec = 0, modeord = 0, ns = 0;
+  int64_t n[3] = {0, 0, 0};  // n_modes (transform) or nf (grid)
+  int64_t n_trans = 0;
+  double eps = 0, upsampfac = 0, beta = 0, kc = 0;
+  void* stream = nullptr;
+  int device = -1;
+  bool operator==(const CacheKey& o) const {
+    return kind == o.kind && type == o.type && dim == o.dim && isign == o.isign && prec == o.prec &&
+           modeord == o.modeord && ns == o.ns && n[0] == o.n[0] && n[1] == o.n[1] && n[2] == o.n[2] &&
+           n_trans == o.n_trans && eps == o.eps && upsampfac == o.upsampfac && beta == o.beta && kc == o.kc &&
+           stream == o.stream && device == o.device;
+  }
+};
+
+struct CacheEntry {
+  CacheKey key;
+  nufftx_plan plan = nullptr;
+  bool busy = false;
+  uint64_t released = 0;  // release tick, for least-recently-released eviction
+};
+
+constexpr int kCacheIdleMax = 32;
+
+struct PlanCache {
+  std::mutex mu;
+  std::vector<CacheEntry> entries;
+  uint64_t tick = 0;
+  int64_t made = 0, reused = 0;
+};
+
+PlanCache& plan_cache() {
+  static PlanCache* c = new PlanCache();  // never destroyed: plans may be released at exit
+  return *c;
+}
+
+int current_device() {
+  int d = -1;
+  if (hipGetDevice(&d) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  return d;
+}
+
+// an idle plan of this key, marked busy; nullptr when there is none
+nufftx_plan cache_take(const CacheKey& k) {
+  PlanCache& pc = plan_cache();
+  std::lock_guard<std::mutex> lk(pc.mu);
+  for (CacheEntry& e : pc.entries)
+    if (!e.busy && e.key == k) {
+      e.busy = true;
+      ++pc.reused;
+      return e.plan;
+    }
+  return nullptr;
+}
+
+void cache_insert_busy(const CacheKey& k, nufftx_plan p) {
+  PlanCache& pc = plan_cache();
+  p->cached = true;
+  std::lock_guard<std::mutex> lk(pc.mu);
+  CacheEntry e;
+  e.key = k;
+  e.plan = p;
+  e.busy = true;
+  pc.entries.push_back(e);
+  ++pc.made;
+}
+
+// Destroy cached plans taken out of the pool (outside the lock).  Their streams may have been
+// destroyed since, so the wait for their work in flight is a synchronisation of their device.
+void cache_destroy(std::vector<CacheEntry>& out) {
+  int cur = -1;
+  const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+  for (CacheEntry& e : out) {
+    if (e.key.device >= 0) (void)hipSetDevice(e.key.device);
+    (void)hipDeviceSynchronize();
+    e.plan->cached = false;
+    (void)nufftx_destroy(e.plan);
+  }
+  if (have_cur && !out.empty()) (void)hipSetDevice(cur);
+  (void)hipGetLastError();
+  out.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int nufftx_plan_acquire(int type, int dim, const int64_t* n_modes, int isign, int64_t n_trans, double eps,
+                        double upsampfac, int precision, int modeord, void* stream, nufftx_plan* out) {
+  if (!out) return fail(NUFFTX_ERR_INVALID_ARG, "plan output pointer is null");
+  *out = nullptr;
+  if (int r = check_plan_args(type, dim, n_modes, n_trans, eps, upsampfac, precision, modeord)) return r;
+  CacheKey k;
+  k.kind = 0;
+  k.type = type;
+  k.dim = dim;
+  // type 1/2 plans serve either sign (the sign is set on the plan at acquire); type 3 bakes it in
+  k.isign = type == 3 ? (isign >= 0 ? 1 : -1) : 0;
+  k.prec = precision;
+  k.modeord = modeord;
+  for (int d = 0; d < dim; ++d) k.n[d] = n_modes[d];
+  k.n_trans = n_trans;
+  k.eps = eps;
+  k.upsampfac = upsampfac;
+  k.stream = stream;
+  k.device = current_device();
+  nufftx_plan p = cache_take(k);
+  if (!p) {
+    if (int r = nufftx_makeplan(type, dim, n_modes, isign, n_trans, eps, upsampfac, precision, modeord, stream, &p))
+      return r;
+    cache_insert_busy(k, p);
+  }
+  p->isign = isign >= 0 ? 1 : -1;
+  *out = p;
+  return NUFFTX_OK;
+}
+
+int nufftx_plan_acquire_grid(int dim, const int64_t* nf, int nspread, double beta, double kc, int64_t n_trans,
+                             int precision, void* stream, nufftx_plan* out) {
+  if (!out) return fail(NUFFTX_ERR_INVALID_ARG, "plan output pointer is null");
+  *out = nullptr;
+  if (int r = check_grid_args(dim, nf, nspread, n_trans, precision)) return r;
+  CacheKey k;
+  k.kind = 1;
+  k.dim = dim;
+  k.prec = precision;
+  k.ns = nspread;
+  for (int d = 0; d < dim; ++d) k.n[d] = nf[d];
+  k.n_trans = n_trans;
+  k.beta = beta;
+  k.kc = kc;
+  k.stream = stream;
+  k.device = current_device();
+  nufftx_plan p = cache_take(k);
+  if (!p) {
+    if (int r = nufftx_makeplan_grid(dim, nf, nspread, beta, kc, n_trans, precision, stream, &p)) return r;
+    cache_insert_busy(k, p);
+  }
+  *out = p;
+  return NUFFTX_OK;
+}
+
+int nufftx_plan_release(nufftx_plan p) {
+  if (!p) return fail(NUFFTX_ERR_INVALID_ARG, "null plan");
+  PlanCache& pc = plan_cache();
+  std::vector<CacheEntry> evict;
+  {
+    std::lock_guard<std::mutex> lk(pc.mu);
+    CacheEntry* mine = nullptr;
+    for (CacheEntry& e : pc.entries)
+      if (e.plan == p) mine = &e;
+    if (!mine) return fail(NUFFTX_ERR_INVALID_ARG, "plan was not acquired from the plan cache");
+    if (!mine->busy) return fail(NUFFTX_ERR_STATE, "plan released twice");
+    mine->busy = false;
+    mine->released = ++pc.tick;
+    int idle = 0;
+    for (const CacheEntry& e : pc.entries) idle += e.busy ? 0 : 1;
+    while (idle > kCacheIdleMax) {
+      size_t lru = pc.entries.size();
+      for (size_t i = 0; i < pc.entries.size(); ++i)
+        if (!pc.entries[i].busy && (lru == pc.entries.size() || pc.entries[i].released < pc.entries[lru].released))
+          lru = i;
+      evict.push_back(pc.entries[lru]);
+      pc.entries.erase(pc.entries.begin() + (std::ptrdiff_t)lru);
+      --idle;
+    }
+  }
+  cache_destroy(evict);
+  return NUFFTX_OK;
+}
+
+int nufftx_plan_cache_clear(void) {
+  PlanCache& pc = plan_cache();
+  std::vector<CacheEntry> idle;
+  {
+    std::lock_guard<std::mutex> lk(pc.mu);
+    std::vector<CacheEntry> keep;
+    for (CacheEntry& e : pc.entries) {
+      if (e.busy)
+        keep.push_back(e);
+      else
+        idle.push_back(e);
+    }
+    pc.entries.swap(keep);
+  }
+  const int n = (int)idle.size();
+  cache_destroy(idle);
+  return n;
+}
+
+int nufftx_plan_cache_stats(int64_t* idle, int64_t* held, int64_t* made, int64_t* reused) {
+  PlanCache& pc = plan_cache();
+  std::lock_guard<std::mutex> lk(pc.mu);
+  int64_t ni = 0, nb = 0;
+  for (const CacheEntry& e : pc.entries) (e.busy ? nb : ni) += 1;
+  if (idle) *idle = ni;
+  if (held) *held = nb;
+  if (made) *made = pc.made;
+  if (reused) *reused = pc.reused;
+  return NUFFTX_OK;
+}
+
+int64_t nufftx_device_allocations(void) { return g_dev_allocs.load(std::memory_order_relaxed); }
 
 }  // extern "C"

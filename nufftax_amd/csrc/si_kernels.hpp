@@ -327,9 +327,9 @@ inline size_t banked_lds(int64_t tile_cells, int maxsub, bool packed = false) {
 // part in its high and the imaginary part in its low 32 bits, both signed fixed point, so a
 // support cell takes one ds_add_u64 instead of two.  Word = re 2^32 + im (mod 2^64) adds
 // exactly while every partial sum stays below 2^31 in magnitude: the scale is 2^(30 - e) with
-// 2^e above 2 n max|c| (n points of the subproblem).  A unit is then at most 2^-19 max|c|
-// for 1024 points, a rounding error near 1e-6 of the strengths, which the eps >= 1e-4 plans
-// that use it do not see.
+// 2^e above 2 n max|c| (n points of the subproblem; frexp gives 2^e <= 4 n max|c|).  A unit is
+// then at most 2^-18 max|c| for 1024 points, a rounding error near 4e-6 of the strengths, which
+// the eps >= 1e-4 plans that use it do not see.
 constexpr double kPackedTol = 1e-4;
 constexpr int kPackedBits = 30;
 

@@ -245,8 +245,7 @@ __global__ void __launch_bounds__(kMfmaThreads) k_spread_mfma2d(SIArgs<float> a,
 
 // cmax[t] = max over j of max(|re c[t][j]|, |im c[t][j]|), as float bits (a NaN or an
 // infinity propagates as the largest pattern): per-block maxima into part[t][block],
-// then one block per transform reduces them.  No atomics.
-constexpr int kAbsmaxBlocks = 1024;
+// then one block per transform reduces them (at most kAbsmaxBlocks partials).  No atomics.
 
 static __global__ void __attribute__((unused)) __launch_bounds__(256) k_absmax_part(const cplx<float>* __restrict__ c,
                                                                                    int64_t M, unsigned* __restrict__ part) {

@@ -461,6 +461,24 @@ __global__ void __launch_bounds__(1024) k_scatter16(const int32_t* __restrict__ 
   }
 }
 
+// Compact records of a 2D point set through the inverse permutation: crec[slot[i]] = the
+// record of point i (coordinates, keys and slots read in index order, 8-byte records
+// scattered).  A pass of its own instead of a write inside the sort's ranking loop: the
+// stream of independent scattered stores keeps more writes in flight (measured on the 2D
+// radial workload, 4M points: the scatter with records took 63 us, the plain scatter and
+// this pass take 27 + 25 us).
+template <typename CT>
+__global__ void __launch_bounds__(256) k_pack_crec(RecArgs ra, const int32_t* __restrict__ keys, int64_t M) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride) {
+    const int pos = ra.slot[i];
+    if ((unsigned)pos >= (unsigned)M) continue;
+    const double x = (double)static_cast<const CT*>(ra.coord[0])[i];
+    const double y = (double)static_cast<const CT*>(ra.coord[1])[i];
+    ra.crec[pos] = compact_record(ra, keys[i], x, y);
+  }
+}
+
 // Packed records of a 2D FP32 point set for the matrix-core window spread: point i of
 // transform t goes to packed[t][slot[i]] = {x word, y word, re c, im c} (compact words as
 // in compact_record, tile-relative to the point's bin from its stored key).  Coordinates,
@@ -542,10 +560,15 @@ void launch_sort_passes(const KeyArgs& ka, PointSet& ps, int nbins, int W, int S
   if (D == 2 && ra.recs)
     hipLaunchKernelGGL((k_scatter<CT, 1>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
                        ps.perm, ra);
-  else if (D == 2 && ra.crec)
+  else if (D == 2 && ra.crec && !ra.slot)
     hipLaunchKernelGGL((k_scatter<CT, 2>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
                        ps.perm, ra);
-  else
+  else if (D == 2 && ra.crec) {
+    hipLaunchKernelGGL((k_scatter<CT, 0>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
+                       ps.perm, ra);
+    const unsigned nb = (unsigned)std::min<int64_t>((ps.M + 255) / 256, 8192);
+    hipLaunchKernelGGL((k_pack_crec<CT>), dim3(nb), dim3(256), 0, s, ra, ps.keys, ps.M);
+  } else
     hipLaunchKernelGGL((k_scatter<CT, 0>), dim3(ps.NB), dim3(64 * W), lds, s, ps.keys, ps.M, SC, nbins, key_bits, sa,
                        ps.perm, ra);
 }
@@ -621,7 +644,8 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   const bool crec = rec_kind == 2 && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
   if (recs && (e = grow_buffer(ps.recs, ps.cap_recs, M, ps.retired)) != hipSuccess) return e;
   if (crec && (e = grow_buffer(ps.crec, ps.cap_crec, M, ps.retired)) != hipSuccess) return e;
-  const bool slot = want_slot && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
+  // the inverse permutation also serves the compact records (written by a pass of their own)
+  const bool slot = (want_slot || crec) && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
   if (slot && (e = grow_buffer(ps.slot, ps.cap_slot, M, ps.retired)) != hipSuccess) return e;
   if (slot && n_trans > 0 &&
       (e = grow_buffer(ps.packed, ps.cap_packed, M * n_trans, ps.retired)) != hipSuccess)
@@ -636,7 +660,7 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   if ((e = grow_buffer(ps.scan_part, ps.cap_part, nparts, ps.retired)) != hipSuccess) return e;
   if ((e = grow_buffer(ps.bin_start, ps.cap_bins, (int64_t)nbins + 1, ps.retired)) != hipSuccess) return e;
   if ((e = grow_buffer(ps.subprob, ps.cap_sub, max_sub, ps.retired)) != hipSuccess) return e;
-  if (!ps.nsub && (e = hipMalloc((void**)&ps.nsub, sizeof(int32_t))) != hipSuccess) return e;
+  if (!ps.nsub && (e = dev_malloc((void**)&ps.nsub, sizeof(int32_t))) != hipSuccess) return e;
   if (nparts > ps.cap_sum) {
     // segment sums start at zero; k_seg_scan leaves them at zero
     if ((e = grow_buffer(ps.scan_sum, ps.cap_sum, nparts, ps.retired)) != hipSuccess) return e;

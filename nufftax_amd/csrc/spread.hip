@@ -49,7 +49,8 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
         // 1024 points per subproblem, the bound the packed scale assumes
         const bool pk = ga.tol >= kPackedTol && ps.maxsub <= 1024 && !std::getenv("NUFFTX_NO_PACKED");
         const size_t lb = banked_lds<D>(a.tile_cells, ps.maxsub, pk);
-        bool fits = ps.maxsub <= 65535 && lb <= (size_t)lds_device_max_bytes();
+        // the static class counters (2 x kBankClasses ints) share the block's LDS
+        bool fits = ps.maxsub <= 65535 && lb + 2 * kBankClasses * sizeof(int) <= (size_t)lds_device_max_bytes();
         for (int d = 0; d < D; ++d) fits = fits && a.bs[d] <= 127;
         if (fits) {
           auto kb = pk ? k_spread_banked<CT, D, NS, false, kBankClasses, true>
@@ -159,7 +160,7 @@ bool banked_ok(const GridOpArgs& ga, size_t& lds) {
   if (!use_tiles(*ga.g, ga.prec, tl, true)) return false;
   const PointSet& ps = *ga.ps;
   lds = banked_lds<D>(ga.g->tile_cells, ps.maxsub);
-  if (ps.maxsub > 65535 || lds > (size_t)lds_device_max_bytes()) return false;
+  if (ps.maxsub > 65535 || lds + 2 * kBankClasses * sizeof(int) > (size_t)lds_device_max_bytes()) return false;
   for (int d = 0; d < D; ++d)
     if (ga.g->bs[d] > 127) return false;
   return true;

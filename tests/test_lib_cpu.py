@@ -54,6 +54,33 @@ def test_version_and_errors_without_gpu():
     assert L.nufftx_grid_size(-1, 2.0, 8) < 0
 
 
+def test_plan_cache_host_checks_without_gpu():
+    """The plan cache's argument checks and bookkeeping run before any device call."""
+    from nufftax_amd import _lib
+
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    nm = (ctypes.c_int64 * 3)(16, 16, 1)
+    assert L.nufftx_plan_acquire(7, 2, nm, 1, 1, 1e-6, 2.0, 0, 0, None, ctypes.byref(h)) == 1
+    assert "type" in _lib.last_error() and not h.value
+    assert L.nufftx_plan_acquire(1, 2, nm, 1, 1, 1e-6, 0.9, 0, 0, None, ctypes.byref(h)) == 1
+    assert L.nufftx_plan_acquire(1, 2, None, 1, 1, 1e-6, 2.0, 0, 0, None, ctypes.byref(h)) == 1
+    assert L.nufftx_plan_acquire(1, 2, nm, 1, 1, 1e-6, 2.0, 0, 0, None, None) == 1
+    assert L.nufftx_plan_acquire_grid(2, nm, 7, 18.0, 0.5, 1, 0, None, ctypes.byref(h)) == 1
+    assert "nspread" in _lib.last_error()
+    assert L.nufftx_plan_acquire_grid(2, nm, 8, 18.0, 0.5, 0, 0, None, ctypes.byref(h)) == 1
+    # release only takes plans that acquire handed out (pointers are compared, never read)
+    assert L.nufftx_plan_release(None) == 1
+    assert L.nufftx_plan_release(ctypes.c_void_p(0x1000)) == 1
+    assert "not acquired" in _lib.last_error()
+    idle, held, made, reused = (ctypes.c_int64() for _ in range(4))
+    assert L.nufftx_plan_cache_stats(ctypes.byref(idle), ctypes.byref(held), ctypes.byref(made),
+                                     ctypes.byref(reused)) == 0
+    assert (idle.value, held.value, made.value, reused.value) == (0, 0, 0, 0)
+    assert L.nufftx_plan_cache_clear() == 0
+    assert L.nufftx_device_allocations() >= 0
+
+
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6, 1e-9, 1e-12, 1e-15])
 @pytest.mark.parametrize("upsampfac", [2.0, 1.25, 1.5])
 def test_kernel_params_match_oracle(eps, upsampfac):
