@@ -85,9 +85,10 @@ struct SortTuning {
 
 // Whether a geometry takes the matrix-core window spread (si_win.hpp): 2D FP32, ns <= 8
 // (window 16 = SB + ns with SB >= 8), square bins of 4 x 4 sub-bins, subproblems of at most
-// 1024 points (two per thread of a 512-thread block)
+// 4096 points (eight per thread of a 512-thread block)
+constexpr int kWinMaxSub = 4096;
 inline bool win2d_geometry(const Geometry& g, int prec, int maxsub) {
-  if (g.dim != 2 || prec != 0 || g.ns > 8 || maxsub > 1024) return false;
+  if (g.dim != 2 || prec != 0 || g.ns > 8 || maxsub > kWinMaxSub) return false;
   const int bs = 4 * (16 - g.ns);
   return g.bs[0] == bs && g.bs[1] == bs && g.tile[0] == bs + g.ns && g.tile[1] == bs + g.ns;
 }

@@ -390,14 +390,18 @@ int check_plan(nufftx_plan p) {
 // interp: the point set is interpolated to (type 2, grid operators, type-3 targets).
 int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g, bool interp, bool spread) {
   std::string err;
-  const bool subsort = subsort_applicable(p->tune, g, p->prec);
+  SortTuning tune = p->tune;
+  // point sets whose forward use is the window spread get subproblems of up to 4096 points
+  // (one block of the window kernel per bin of the 2D radial workload)
+  if (spread && win2d_geometry(g, p->prec, kWinMaxSub) && !std::getenv("NUFFTX_MAXSUB")) tune.maxsub = kWinMaxSub;
+  const bool subsort = subsort_applicable(tune, g, p->prec);
   // records pay for their cost in the sort only where the point set is interpolated;
   // the matrix-core kernels need the full records with sub-bins
   const int rec_kind = g.dim == 2 && p->prec == kF32 ? (subsort ? 1 : interp ? 2 : 0) : 0;
   // point sets that the matrix-core window kernel spreads get the inverse permutation (and,
   // when spreading is their forward use, the packed-record buffer)
-  const bool slot = win2d_geometry(g, p->prec, p->tune.maxsub);
-  hipError_t e = pointset_sort(ps, g, p->tune, p->stream, err, rec_kind, subsort ? 16 - g.ns : 0, slot,
+  const bool slot = win2d_geometry(g, p->prec, tune.maxsub);
+  hipError_t e = pointset_sort(ps, g, tune, p->stream, err, rec_kind, subsort ? 16 - g.ns : 0, slot,
                                spread ? p->n_trans : 0);
   if (e != hipSuccess) {
     if (!err.empty()) return fail(NUFFTX_ERR_INVALID_ARG, err);

@@ -341,6 +341,296 @@ __global__ void __launch_bounds__(kWinThreads) k_spread_win2d(SIArgs<float> a, i
   }
 }
 
+// ---- fp16-split window spread
+//
+// The same windows, with the matrix products on v_mfma_f32_16x16x32_f16 (32 points per
+// instruction, measured at 18 clocks on MI355X against 36 for v_mfma_f32_16x16x4f32 with
+// 4 points; VALU and MFMA instructions of one SIMD do not run concurrently, so MFMA clocks
+// add to the kernel time).  FP32 operands are split into two fp16 parts, v = hi + lo with
+// hi = fp16(v) and lo = fp16(v - hi), and each window accumulates
+//   A_hi B_hi + A_lo B_hi + A_hi B_lo          (three MFMAs per part, FP32 accumulation),
+// which leaves out A_lo B_lo and the rounding of lo: at most 2^-22 |A| |B| per term.  Both
+// operands are scaled to the top of fp16's range, so that lo parts stay normal numbers (the
+// matrix cores flush fp16 subnormal inputs: with B = wx in [0, 1] the lo parts of weights
+// below 2^-3 were lost, a 5e-6 relative error on the 2D radial workload):
+// A = ldexp(c, 15 - e) wy with 2^e above the subproblem's max |re c|, |im c| (|A| < 2^15) and
+// B = 2^14 wx (B < 2^15); lo is then normal down to |A|, B = 2^-3, i.e. relative to the
+// largest operand 2^-18, and the terms lost below that are under 2^-29 of it.
+//
+// Subproblems hold up to 4096 points (8 per thread while ordering), so one block covers a
+// bin of the 2D radial workload where the FP32 kernel needed four: the tile flush to the
+// fine grid and the window flushes are paid four times less often.  Sub-bins are padded to
+// multiples of 32 points (zero-strength entries).  Tiles are exact (two 64-bit words per
+// cell, scale 2^(50 - e) with 2^e above 2 n max(|re c|, |im c|)).
+constexpr int kWhThreads = 512;                  // 8 waves per subproblem
+constexpr int kWhPpt = 8;                        // points per thread while ordering
+constexpr int kWhMaxPts = kWhThreads * kWhPpt;   // 4096 points per subproblem
+static_assert(kWhMaxPts == kWinMaxSub, "win2d_geometry admits subproblems of up to kWinMaxSub points");
+constexpr int kWhK = 32;                         // points per group (K of the f16 MFMA)
+constexpr int kWhRowF = kWhK * kWinRow + 16;     // floats of one row block (+ trailing zeros)
+constexpr int kWhWaveF = 2 * kWhRowF + 4 * kWhK; // floats per wave: x rows, y rows, meta
+
+typedef _Float16 wh_h8 __attribute__((ext_vector_type(8)));
+
+__host__ __device__ inline size_t wh_order_bytes(int maxsub) {
+  return ((size_t)(maxsub + 16 * (kWhK - 1)) * sizeof(uint16_t) + 15) / 16 * 16;
+}
+
+// LDS bytes of the fp16-split window spread (dynamic part) for subproblems of maxsub points
+template <int NS>
+inline size_t wh_spread_lds(int maxsub) {
+  constexpr int T = WinGeom<NS>::T;
+  return 2 * (size_t)T * T * sizeof(unsigned long long) + wh_order_bytes(maxsub) +
+         (size_t)(kWhThreads / 64) * kWhWaveF * sizeof(float);
+}
+
+// v = hi + lo (fp16 parts, round to nearest)
+__device__ __forceinline__ void split_f16(const float (&v)[8], wh_h8& hi, wh_h8& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const _Float16 h = (_Float16)v[e];
+    hi[e] = h;
+    lo[e] = (_Float16)(v[e] - (float)h);
+  }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kWhThreads) k_spread_win2dh(SIArgs<float> a, int maxsub,
+                                                              const uint4* __restrict__ packed,
+                                                              cplx<float>* __restrict__ fw) {
+  using G = WinGeom<NS>;
+  using Word = unsigned long long;
+  constexpr int SB = G::SB, T = G::T, TC = T * T;
+  constexpr int W = kWhThreads / 64;
+  constexpr uint16_t kPad = 0xFFFF;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Word* sre = reinterpret_cast<Word*>(smem_raw);
+  Word* sim = sre + TC;
+  uint16_t* order = reinterpret_cast<uint16_t*>(sim + TC);
+  __shared__ unsigned short wcnt[kWhPpt * W * 16];  // per (pass, wave, sub-bin): count, then first slot
+  __shared__ int cnt[16], start[17];
+  __shared__ unsigned smax;
+  const int sub = blockIdx.x;
+  if (sub >= *a.nsub) return;
+  const int tr = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int4 sd = a.subprob[sub];
+  const int n = sd.z;
+  const uint4* __restrict__ pk = packed + (int64_t)tr * a.M + sd.y;
+  float* const lf = reinterpret_cast<float*>(smem_raw);
+  const int wof = (int)((2 * (size_t)TC * sizeof(Word) + wh_order_bytes(maxsub)) / 4) + wave * kWhWaveF;
+  float* rowx = lf + wof;         // x weights: row q at 17 q, weights at 17 q + 16 - NS .. 17 q + 15
+  float* rowy = rowx + kWhRowF;   // y weights, same layout
+  int4* meta = reinterpret_cast<int4*>(rowy + kWhRowF);  // {y read index, x read index, re A, im A}
+  int org[3];
+  tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
+  for (int e = tid; e < TC; e += kWhThreads) {
+    sre[e] = 0;
+    sim[e] = 0;
+  }
+  for (int e = lane; e < kWhRowF; e += 64)  // the zero parts of the weight rows stay zero
+    if (e % kWinRow < 16 - NS || e % kWinRow == 16 || e >= kWhK * kWinRow) {
+      rowx[e] = 0.0f;
+      rowy[e] = 0.0f;
+    }
+  for (int e = tid; e < kWhPpt * W * 16; e += kWhThreads) wcnt[e] = 0;
+  if (tid == 0) smax = 0u;
+  __syncthreads();
+
+  // ---- 1. order (as the FP32 kernel, 8 points per thread, sub-bins padded to 32)
+  int sb[kWhPpt];
+  unsigned mymax = 0;
+#pragma unroll
+  for (int r = 0; r < kWhPpt; ++r) {
+    const int j = tid + r * kWhThreads;
+    sb[r] = -1;
+    if (j < n) {
+      const uint4 p = pk[j];
+      const int sx = min((int)(p.x >> 25) / SB, 3), sy = min((int)(p.y >> 25) / SB, 3);
+      const float are = fabsf(__uint_as_float(p.z)), aim = fabsf(__uint_as_float(p.w));
+      mymax = max(mymax, max(__float_as_uint(are), __float_as_uint(aim)));
+      sb[r] = sy * 4 + sx;
+    }
+  }
+  int rank[kWhPpt];
+#pragma unroll
+  for (int r = 0; r < kWhPpt; ++r) {
+    const bool valid = sb[r] >= 0;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint64_t bal = __ballot(valid && ((sb[r] >> b) & 1));
+      m &= ((sb[r] >> b) & 1) ? bal : ~bal;
+    }
+    rank[r] = popc_below(m);
+    if (valid && rank[r] == 0) wcnt[(r * W + wave) * 16 + sb[r]] = (unsigned short)__popcll(m);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mymax = max(mymax, (unsigned)__shfl_xor((int)mymax, off, 64));
+  if (lane == 0) atomicMax(&smax, mymax);
+  __syncthreads();
+  if (tid < 16) {
+    int c = 0;
+    for (int e = 0; e < kWhPpt * W; ++e) c += wcnt[e * 16 + tid];
+    cnt[tid] = c;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int s = 0; s < 16; ++s) {
+      start[s] = acc;
+      acc += (cnt[s] + kWhK - 1) & ~(kWhK - 1);
+    }
+    start[16] = acc;
+  }
+  __syncthreads();
+  if (tid < 16) {
+    int run = start[tid];
+    for (int e = 0; e < kWhPpt * W; ++e) {
+      const int v = wcnt[e * 16 + tid];
+      wcnt[e * 16 + tid] = (unsigned short)run;
+      run += v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kWhPpt; ++r)
+    if (sb[r] >= 0) order[wcnt[(r * W + wave) * 16 + sb[r]] + rank[r]] = (uint16_t)(tid + r * kWhThreads);
+  if (tid < 16 * (kWhK - 1)) {  // zero-strength padding entries
+    const int s = tid / (kWhK - 1), q = tid % (kWhK - 1);
+    const int slot = start[s] + cnt[s] + q;
+    if (slot < start[s + 1]) order[slot] = kPad;
+  }
+  const float cm = __uint_as_float(smax);
+  const double B = 2.0 * (double)n * (double)cm;
+  const bool finite = isfinite(B) && isfinite(cm);
+  double scale = 1.0, inv_scale = 1.0, wscale = 1.0;
+  int ke = 0;  // strengths scale into fp16's range: A = ldexp(c, ke) wy, |A| < 2^15
+  if (finite && B > 0.0) {
+    int e;
+    (void)frexp(B, &e);  // B < 2^e
+    scale = ldexp(1.0, 50 - e);
+    inv_scale = ldexp(1.0, e - 50);
+    int ec;
+    (void)frexpf(cm, &ec);  // cm < 2^ec
+    ke = 15 - ec;
+    wscale = ldexp(scale, -ke - 14);  // window values are in units of 2^-(ke + 14)
+  }
+  __syncthreads();
+
+  // ---- 2. accumulate: groups of 32 points of one sub-bin, contiguous ranges per wave
+  const int NG = start[16] / kWhK;
+  const int g0 = (int)(((int64_t)NG * wave) / W), g1 = (int)(((int64_t)NG * (wave + 1)) / W);
+  const int i = lane & 15, kq = lane >> 4;
+  const int sq = lane & (kWhK - 1), sdim = lane >> 5;  // weight staging: point sq, dimension sdim
+  const float bl2e = a.es.beta_log2e, kc = a.es.c;
+  float* const myrow = (sdim ? rowy : rowx) + kWinRow * sq + (16 - NS);
+  const int ry0 = wof + kWhRowF + (16 - NS), rx0 = wof + (16 - NS);  // first weight of row 0 (y, x)
+  // the matrix cores align each product to the largest addend and truncate it there, which
+  // biases a long accumulation; so every group's three MFMAs start from zero (t) and the group
+  // sums add into the window in FP32 on the VALU (round to nearest), one group later, when
+  // the MFMAs have long finished
+  const win_f4 zero4 = {0.f, 0.f, 0.f, 0.f};
+  win_f4 acc_re = zero4, acc_im = zero4, t_re = zero4, t_im = zero4;
+  int s = 0;
+  while (s < 15 && start[s + 1] <= kWhK * g0) ++s;
+  int next = start[s + 1];
+  auto flush = [&](int sbin) {
+    const int x = (sbin & 3) * SB + i;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int cell = ((sbin >> 2) * SB + 4 * kq + rr) * T + x;
+      atomicAdd(&sre[cell], TileAcc<float>::quant((double)acc_re[rr], wscale));
+      atomicAdd(&sim[cell], TileAcc<float>::quant((double)acc_im[rr], wscale));
+    }
+  };
+  auto fetch = [&](int g) {
+    const int j = g < g1 ? order[kWhK * g + sq] : kPad;
+    return j != kPad ? pk[j] : make_uint4(0u, 0u, 0u, 0u);
+  };
+  uint4 p = fetch(g0);
+  for (int g = g0; g < g1 && finite; ++g) {
+    acc_re += t_re;
+    acc_im += t_im;
+    if (kWhK * g >= next) {  // the group opens the next non-empty sub-bin
+      flush(s);
+      acc_re = zero4;
+      acc_im = zero4;
+      do {
+        ++s;
+      } while (s < 15 && start[s + 1] <= kWhK * g);
+      next = start[s + 1];
+    }
+    // -- kernel weights of the group's points (lanes 0-31 x, 32-63 y)
+    {
+      const unsigned wd = sdim ? p.y : p.x;
+      const float t = (float)((double)(wd & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1));
+      const float wsc = sdim ? 1.0f : 16384.0f;  // B = 2^14 wx
+#pragma unroll
+      for (int m = 0; m < NS; ++m) myrow[m] = es_phi(t + (float)m, bl2e, kc) * wsc;
+      if (sdim == 0) {
+        const int lx = (int)(p.x >> 25), ly = (int)(p.y >> 25);
+        const int ox = lx - min(lx / SB, 3) * SB, oy = ly - min(ly / SB, 3) * SB;  // offsets in the window
+        meta[sq] = make_int4(ry0 + kWinRow * sq - oy, rx0 + kWinRow * sq - ox,
+                             __float_as_int(ldexpf(__uint_as_float(p.z), ke)),
+                             __float_as_int(ldexpf(__uint_as_float(p.w), ke)));
+      }
+    }
+    p = fetch(g + 1);
+    wave_sync_win();
+    // -- operands: lane (i, kq) holds A[i][8 kq + e] and B[8 kq + e][i]
+    float are[8], aim[8], bx[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int4 mt = meta[8 * kq + e];
+      const float wy = lf[mt.x + i];
+      bx[e] = lf[mt.y + i];
+      are[e] = __int_as_float(mt.z) * wy;
+      aim[e] = __int_as_float(mt.w) * wy;
+      // the split below must see one rounded value: the empty asm keeps the compiler from
+      // fusing the product into one use (hi from the exact product) and not the other (hi
+      // from the rounded one), which left hi + lo a whole fp16 ulp off now and then
+      asm volatile("" : "+v"(are[e]), "+v"(aim[e]), "+v"(bx[e]));
+    }
+    wave_sync_win();
+    wh_h8 bh, bl, ah, al;
+    split_f16(bx, bh, bl);
+    split_f16(are, ah, al);
+    t_re = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, zero4, 0, 0, 0);
+    t_re = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, t_re, 0, 0, 0);
+    t_re = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, t_re, 0, 0, 0);
+    split_f16(aim, ah, al);
+    t_im = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, zero4, 0, 0, 0);
+    t_im = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, t_im, 0, 0, 0);
+    t_im = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, t_im, 0, 0, 0);
+  }
+  acc_re += t_re;
+  acc_im += t_im;
+  if (g1 > g0 && finite) flush(s);
+  __syncthreads();
+
+  // ---- 3. flush the tile's non-zero cells to the fine grid
+  cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
+  const int tile3[3] = {T, T, 1};
+  for (int cell = tid; cell < TC; cell += kWhThreads) {
+    const long long ir = (long long)sre[cell], ii = (long long)sim[cell];
+    if (finite && ir == 0 && ii == 0) continue;
+    const float vr = finite ? (float)((double)ir * inv_scale) : NAN;
+    const float vi = finite ? (float)((double)ii * inv_scale) : NAN;
+    const int64_t gi = tile_cell_to_global<2>(cell, tile3, org, a.nf);
+    atomic_add_global(&fT[gi].re, vr);
+    atomic_add_global(&fT[gi].im, vi);
+  }
+}
+
+// NUFFTX_WIN_F32=1: the FP32 window kernel (v_mfma_f32_16x16x4f32, subproblems of at most
+// 1024 points) instead of the fp16-split one
+inline bool win_f32_kernel() {
+  const char* e = std::getenv("NUFFTX_WIN_F32");
+  return e != nullptr && e[0] == '1';
+}
+
 // NUFFTX_WIN_EXACT=1: two 64-bit words per tile cell (exact tile sums) instead of packed tiles
 inline bool win_exact_tiles() {
   const char* e = std::getenv("NUFFTX_WIN_EXACT");

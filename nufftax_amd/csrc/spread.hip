@@ -109,7 +109,7 @@ bool spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, h
   return false;
 }
 
-// The matrix-core window spread (2D FP32, ns <= 8, subproblems of <= 1024 points): the
+// The matrix-core window spread (2D FP32, ns <= 8, subproblems of <= 4096 points): the
 // points are packed in sorted order first (the sort left the inverse permutation), then
 // spread from those contiguous records.
 bool spread_win(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hipError_t& err) {
@@ -118,6 +118,18 @@ bool spread_win(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hi
   SIArgs<float> a = make_si_args<float>(ga);
   auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;
+    if (!win_f32_kernel() || ps.maxsub > 1024) {
+      // the fp16-split kernel; its static LDS (counters) is below 2.2 KB
+      const size_t lds = wh_spread_lds<NS>(ps.maxsub);
+      if (lds + 2304 > (size_t)lds_device_max_bytes()) return false;
+      if ((err = pointset_pack2d(ps, *ga.g, c, ga.n_trans, s)) != hipSuccess) return true;
+      auto kern = k_spread_win2dh<NS>;
+      if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kWhThreads), lds, s, a, ps.maxsub,
+                         ps.packed, static_cast<cplx<float>*>(fw));
+      err = hipGetLastError();
+      return true;
+    }
     const bool exact = win_exact_tiles();
     const size_t lds = exact ? win_spread_lds<NS, false>(ps.maxsub) : win_spread_lds<NS, true>(ps.maxsub);
     if (lds + 2048 > (size_t)lds_device_max_bytes()) return false;
