@@ -80,6 +80,15 @@ __device__ __forceinline__ float es_phi(float z, float beta_log2e, float c) {
   return cz2 <= 1.0f ? v : 0.0f;
 }
 
+// es_phi for arguments inside the support (c z^2 <= 1 up to rounding): the same value there,
+// without the range select (outside, the clamp gives exp(-beta c z^2) instead of 0)
+__device__ __forceinline__ float es_phi_in(float z, float beta_log2e, float c) {
+  const float cz2 = c * z * z;
+  const float a = fmaxf(1.0f - cz2, 0.0f);
+  const float s = __builtin_amdgcn_sqrtf(a);
+  return __builtin_amdgcn_exp2f(-beta_log2e * cz2 * __builtin_amdgcn_rcpf(1.0f + s));
+}
+
 __device__ __forceinline__ double es_phi(double z, double beta, double c) {
   double cz2 = c * z * z;
   double a = 1.0 - cz2;

@@ -384,14 +384,25 @@ inline size_t wh_spread_lds(int maxsub) {
          (size_t)(kWhThreads / 64) * kWhWaveF * sizeof(float);
 }
 
-// v = hi + lo (fp16 parts, round to nearest)
+// v = hi + lo (fp16 parts).  lo must be taken from the very hi that the caller uses: the
+// empty asm makes hi one opaque value, so that the compiler cannot rebuild it in another
+// way for the subtraction (a hi from the exact product there and from the rounded one in the
+// operand left hi + lo a whole fp16 ulp off now and then).
 __device__ __forceinline__ void split_f16(const float (&v)[8], wh_h8& hi, wh_h8& lo) {
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const _Float16 h = (_Float16)v[e];
-    hi[e] = h;
-    lo[e] = (_Float16)(v[e] - (float)h);
-  }
+  for (int e = 0; e < 8; ++e) hi[e] = (_Float16)v[e];
+  asm("" : "+v"(hi));
+#pragma unroll
+  for (int e = 0; e < 8; ++e) lo[e] = (_Float16)(v[e] - (float)hi[e]);
+}
+
+// a b = hi + lo for the products of a and b (lo from the exact product, one rounding)
+__device__ __forceinline__ void split_prod_f16(const float (&a)[8], const float (&b)[8], wh_h8& hi, wh_h8& lo) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) hi[e] = (_Float16)(a[e] * b[e]);
+  asm("" : "+v"(hi));
+#pragma unroll
+  for (int e = 0; e < 8; ++e) lo[e] = (_Float16)__builtin_fmaf(a[e], b[e], -(float)hi[e]);
 }
 
 template <int NS>
@@ -567,8 +578,10 @@ __global__ void __launch_bounds__(kWhThreads) k_spread_win2dh(SIArgs<float> a, i
       const unsigned wd = sdim ? p.y : p.x;
       const float t = (float)((double)(wd & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1));
       const float wsc = sdim ? 1.0f : 16384.0f;  // B = 2^14 wx
+      // t + m lies in [-NS/2, NS/2] for the points (t in [-NS/2, 1 - NS/2)); padding entries
+      // (t = -NS/2 - 1, zero strength) get finite weights
 #pragma unroll
-      for (int m = 0; m < NS; ++m) myrow[m] = es_phi(t + (float)m, bl2e, kc) * wsc;
+      for (int m = 0; m < NS; ++m) myrow[m] = es_phi_in(t + (float)m, bl2e, kc) * wsc;
       if (sdim == 0) {
         const int lx = (int)(p.x >> 25), ly = (int)(p.y >> 25);
         const int ox = lx - min(lx / SB, 3) * SB, oy = ly - min(ly / SB, 3) * SB;  // offsets in the window
@@ -580,27 +593,23 @@ __global__ void __launch_bounds__(kWhThreads) k_spread_win2dh(SIArgs<float> a, i
     p = fetch(g + 1);
     wave_sync_win();
     // -- operands: lane (i, kq) holds A[i][8 kq + e] and B[8 kq + e][i]
-    float are[8], aim[8], bx[8];
+    float cre[8], cim[8], wy[8], bx[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int4 mt = meta[8 * kq + e];
-      const float wy = lf[mt.x + i];
+      wy[e] = lf[mt.x + i];
       bx[e] = lf[mt.y + i];
-      are[e] = __int_as_float(mt.z) * wy;
-      aim[e] = __int_as_float(mt.w) * wy;
-      // the split below must see one rounded value: the empty asm keeps the compiler from
-      // fusing the product into one use (hi from the exact product) and not the other (hi
-      // from the rounded one), which left hi + lo a whole fp16 ulp off now and then
-      asm volatile("" : "+v"(are[e]), "+v"(aim[e]), "+v"(bx[e]));
+      cre[e] = __int_as_float(mt.z);
+      cim[e] = __int_as_float(mt.w);
     }
     wave_sync_win();
     wh_h8 bh, bl, ah, al;
     split_f16(bx, bh, bl);
-    split_f16(are, ah, al);
+    split_prod_f16(cre, wy, ah, al);
     t_re = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, zero4, 0, 0, 0);
     t_re = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, t_re, 0, 0, 0);
     t_re = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, t_re, 0, 0, 0);
-    split_f16(aim, ah, al);
+    split_prod_f16(cim, wy, ah, al);
     t_im = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, zero4, 0, 0, 0);
     t_im = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, t_im, 0, 0, 0);
     t_im = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, t_im, 0, 0, 0);
