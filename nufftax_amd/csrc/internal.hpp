@@ -117,8 +117,10 @@ SortSizes sort_sizes(int64_t M, const Geometry& g, const SortTuning& tune, int r
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
                          int rec_kind = 0, int sub_sb = 0, bool want_slot = false, int64_t n_trans = 1);
 // packed[t][slot[i]] = {x word, y word, c[t][i]} of a 2D point set sorted with want_slot (FP32
-// strengths; a non-finite coordinate makes its strength NaN)
-hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s);
+// strengths; a non-finite coordinate makes its strength NaN); the same launch clears
+// zero_bytes (a multiple of 16) at zero, when given
+hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s,
+                           void* zero = nullptr, int64_t zero_bytes = 0);
 
 // Device buffer of at least `need` elements (contents not kept when it grows).  The
 // outgrown buffer is not freed here: hipFree waits for the whole device, which would
@@ -157,8 +159,10 @@ struct GridOpArgs {
   int64_t n_trans;
   double beta, c;
   double tol = 0.0;  // the plan's requested accuracy (0: unknown), selects packed FP32 tiles
+  bool zero_out = false;  // launch_spread overwrites fw instead of adding to it
 };
-// fw[n_trans][grid] += spread(c[n_trans][M]); fw must be zeroed by the caller.
+// fw[n_trans][grid] += spread(c[n_trans][M]) (fw = ... with zero_out: the clear then rides on
+// the window path's packing pass, or is a fill kernel of its own on the other paths).
 hipError_t launch_spread(const GridOpArgs& a, const void* c, void* fw, hipStream_t s);
 // c[n_trans][M] = interp(fw)
 hipError_t launch_interp(const GridOpArgs& a, const void* fw, void* c, hipStream_t s);

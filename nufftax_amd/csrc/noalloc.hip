@@ -72,18 +72,14 @@ extern "C" int nufftx_spread_noalloc(int dim, const int64_t* nf, int nspread, do
   if (nspread < 2 || nspread > 16 || (nspread & 1)) return fail(NUFFTX_ERR_INVALID_ARG, "nspread must be even in [2, 16]");
   if (M < 0 || M > NUFFTX_NOALLOC_MAX_POINTS)
     return fail(NUFFTX_ERR_INVALID_ARG, "nufftx_spread_noalloc: M above NUFFTX_NOALLOC_MAX_POINTS");
-  int64_t cells = 1;
-  for (int d = 0; d < dim; ++d) {
+  for (int d = 0; d < dim; ++d)
     if (nf[d] < 1 || nf[d] > (1LL << 30)) return fail(NUFFTX_ERR_INVALID_ARG, "nf out of range");
-    cells *= nf[d];
-  }
   if (!fw) return fail(NUFFTX_ERR_INVALID_ARG, "fw is null");
   const void* xs[3] = {x, y, z};
   for (int d = 0; d < dim; ++d)
     if (M > 0 && !xs[d]) return fail(NUFFTX_ERR_INVALID_ARG, "missing coordinate array");
   if (M > 0 && !c) return fail(NUFFTX_ERR_INVALID_ARG, "c is null");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int csize = precision == NUFFTX_F32 ? 8 : 16;
 
   Geometry g;
   g.dim = dim;
@@ -130,9 +126,8 @@ extern "C" int nufftx_spread_noalloc(int dim, const int64_t* nf, int nspread, do
     return fail(NUFFTX_ERR_HIP, std::string("nufftx_spread_noalloc bin-sort: ") + hipGetErrorString(e));
   }
   if (!ps.retired.empty()) return fail(NUFFTX_ERR_STATE, "nufftx_spread_noalloc: a buffer outgrew its scratch slice");
-  if ((e = launch_zero(fw, cells * csize, s)) != hipSuccess)
-    return fail(NUFFTX_ERR_HIP, std::string("nufftx_spread_noalloc: ") + hipGetErrorString(e));
   GridOpArgs ga;
+  ga.zero_out = true;  // fw = spread(c)
   ga.ps = &ps;
   ga.g = &g;
   ga.prec = precision;

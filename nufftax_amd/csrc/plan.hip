@@ -475,9 +475,9 @@ int modes_to_grid(nufftx_plan p, const void* f, int isign) {
 }
 
 int exec_type1(nufftx_plan p, const void* c, void* f, int isign) {
-  const int64_t gbytes = p->g.grid_cells() * p->n_trans * (int64_t)p->csize();
-  LAUNCH(launch_zero(p->grid, gbytes, p->stream));
-  LAUNCH(launch_spread(gop(p, p->ps, p->g), c, p->grid, p->stream));
+  GridOpArgs ga = gop(p, p->ps, p->g);
+  ga.zero_out = true;
+  LAUNCH(launch_spread(ga, c, p->grid, p->stream));
   return grid_to_modes(p, f, isign);
 }
 
@@ -489,9 +489,9 @@ int exec_type2(nufftx_plan p, void* c, const void* f, int isign) {
 
 int exec_type3(nufftx_plan p, const void* c, void* f) {
   LAUNCH(launch_cmul_pointwise(p->prec, p->n_trans, p->M, c, p->pre, p->cbuf, p->stream));
-  const int64_t gbytes = p->g.grid_cells() * p->n_trans * (int64_t)p->csize();
-  LAUNCH(launch_zero(p->grid, gbytes, p->stream));
-  LAUNCH(launch_spread(gop(p, p->ps, p->g), p->cbuf, p->grid, p->stream));
+  GridOpArgs ga = gop(p, p->ps, p->g);
+  ga.zero_out = true;
+  LAUNCH(launch_spread(ga, p->cbuf, p->grid, p->stream));
   // inner type 2 on the spread grid as CMCL modes (nufft3.py:292, 387, 494-502)
   LAUNCH(launch_deconv_pad(dargs(p, true), p->grid, p->grid_in, p->stream));
   if (int r = fft_exec(p->fft_in, p->grid_in, p->isign, p->stream)) return r;
@@ -815,8 +815,9 @@ int nufftx_spread(nufftx_plan p, const void* c, void* fw) {
   if (int r = check_plan(p)) return r;
   if (!p->pts_set) return fail(NUFFTX_ERR_STATE, "setpts has not been called");
   if (!fw || (!c && p->M > 0)) return fail(NUFFTX_ERR_INVALID_ARG, "null data pointer");
-  LAUNCH(launch_zero(fw, p->g.grid_cells() * p->n_trans * (int64_t)p->csize(), p->stream));
-  LAUNCH(launch_spread(gop(p, p->ps, p->g), c, fw, p->stream));
+  GridOpArgs ga = gop(p, p->ps, p->g);
+  ga.zero_out = true;
+  LAUNCH(launch_spread(ga, c, fw, p->stream));
   return NUFFTX_OK;
 }
 

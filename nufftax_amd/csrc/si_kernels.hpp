@@ -814,12 +814,13 @@ __global__ void __launch_bounds__(256) k_gather_wide(SIArgs<float> a, const cplx
         t[d] = (float)((double)i0 - f.xs);
       }
     }
+    // t + k lies in [-NS/2, NS/2] (t in [-NS/2, 1 - NS/2)): no range select needed
     float wx[NS], wy[NS], wz[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-      wx[k] = es_phi(t[0] + (float)k, a.es.beta_log2e, a.es.c);
-      wy[k] = es_phi(t[1] + (float)k, a.es.beta_log2e, a.es.c);
-      wz[k] = D == 3 ? es_phi(t[2] + (float)k, a.es.beta_log2e, a.es.c) : 1.0f;
+      wx[k] = es_phi_in(t[0] + (float)k, a.es.beta_log2e, a.es.c);
+      wy[k] = es_phi_in(t[1] + (float)k, a.es.beta_log2e, a.es.c);
+      wz[k] = D == 3 ? es_phi_in(t[2] + (float)k, a.es.beta_log2e, a.es.c) : 1.0f;
     }
     const bool odd = (l[0] & 1) != 0;
     float ws[NC];

@@ -486,8 +486,11 @@ __global__ void __launch_bounds__(256) k_pack_crec(RecArgs ra, const int32_t* __
 template <typename CT>
 __global__ void __launch_bounds__(256) k_pack2d(RecArgs ra, const int32_t* __restrict__ keys, int64_t M,
                                                 int64_t n_trans, const float2* __restrict__ c,
-                                                uint4* __restrict__ packed) {
+                                                uint4* __restrict__ packed, uint4* __restrict__ zero, int64_t nzero) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // the output grid of the spread that follows, cleared here instead of by a launch of its own
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += stride)
+    zero[i] = make_uint4(0u, 0u, 0u, 0u);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride) {
     const int pos = ra.slot[i];
     if ((unsigned)pos >= (unsigned)M) continue;
@@ -710,8 +713,9 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   return e;
 }
 
-hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s) {
-  if (!ps.has_slot || g.dim != 2) return hipErrorInvalidValue;
+hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s,
+                           void* zero, int64_t zero_bytes) {
+  if (!ps.has_slot || g.dim != 2 || zero_bytes % 16 != 0) return hipErrorInvalidValue;
   if (ps.M == 0) return hipSuccess;
   // sized by setpts for the plans that spread; a point set that is only interpolated
   // (type 2) gets its buffer on its first spread (the reverse-mode derivative)
@@ -727,13 +731,14 @@ hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64
   ra.nb0 = g.nb[0];
   ra.ns = g.ns;
   ra.slot = ps.slot;
-  const unsigned nb = (unsigned)std::min<int64_t>((ps.M + 255) / 256, 8192);
+  const int64_t nzero = zero ? zero_bytes / 16 : 0;
+  const unsigned nb = (unsigned)std::min<int64_t>((std::max<int64_t>(ps.M, nzero) + 255) / 256, 8192);
   if (ps.coord_f64)
     hipLaunchKernelGGL(k_pack2d<double>, dim3(nb), dim3(256), 0, s, ra, ps.keys, ps.M, n_trans,
-                       static_cast<const float2*>(c), ps.packed);
+                       static_cast<const float2*>(c), ps.packed, static_cast<uint4*>(zero), nzero);
   else
     hipLaunchKernelGGL(k_pack2d<float>, dim3(nb), dim3(256), 0, s, ra, ps.keys, ps.M, n_trans,
-                       static_cast<const float2*>(c), ps.packed);
+                       static_cast<const float2*>(c), ps.packed, static_cast<uint4*>(zero), nzero);
   return hipGetLastError();
 }
 

@@ -116,13 +116,20 @@ bool spread_win(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hi
   PointSet& ps = const_cast<PointSet&>(*ga.ps);
   if (!win2d_geometry(*ga.g, ga.prec, ps.maxsub) || !ps.has_slot || !use_win_spread()) return false;
   SIArgs<float> a = make_si_args<float>(ga);
+  // zero_out: the packing pass clears the grid (n_trans grids of complex64; an odd cell count
+  // leaves 8 bytes over the pass's 16-byte stores, and takes the fill kernel)
+  const int64_t grid_bytes = ga.n_trans * ga.g->grid_cells() * (int64_t)sizeof(cplx<float>);
+  const bool fused_zero = ga.zero_out && grid_bytes % 16 == 0;
+  void* zero = fused_zero ? fw : nullptr;
+  const int64_t zero_bytes = fused_zero ? grid_bytes : 0;
   auto launch = [&](auto nsc) -> bool {
     constexpr int NS = decltype(nsc)::value;
     if (!win_f32_kernel() || ps.maxsub > 1024) {
       // the fp16-split kernel; its static LDS (counters) is below 2.2 KB
       const size_t lds = wh_spread_lds<NS>(ps.maxsub);
       if (lds + 2304 > (size_t)lds_device_max_bytes()) return false;
-      if ((err = pointset_pack2d(ps, *ga.g, c, ga.n_trans, s)) != hipSuccess) return true;
+      if (ga.zero_out && !fused_zero && (err = launch_zero(fw, grid_bytes, s)) != hipSuccess) return true;
+      if ((err = pointset_pack2d(ps, *ga.g, c, ga.n_trans, s, zero, zero_bytes)) != hipSuccess) return true;
       auto kern = k_spread_win2dh<NS>;
       if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kWhThreads), lds, s, a, ps.maxsub,
@@ -133,7 +140,8 @@ bool spread_win(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hi
     const bool exact = win_exact_tiles();
     const size_t lds = exact ? win_spread_lds<NS, false>(ps.maxsub) : win_spread_lds<NS, true>(ps.maxsub);
     if (lds + 2048 > (size_t)lds_device_max_bytes()) return false;
-    if ((err = pointset_pack2d(ps, *ga.g, c, ga.n_trans, s)) != hipSuccess) return true;
+    if (ga.zero_out && !fused_zero && (err = launch_zero(fw, grid_bytes, s)) != hipSuccess) return true;
+    if ((err = pointset_pack2d(ps, *ga.g, c, ga.n_trans, s, zero, zero_bytes)) != hipSuccess) return true;
     auto kern = exact ? k_spread_win2d<NS, false> : k_spread_win2d<NS, true>;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kWinThreads), lds, s, a, ps.maxsub,
@@ -154,8 +162,24 @@ template <typename T, typename CT>
 hipError_t spread_dim(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     hipError_t err = hipSuccess;
-    if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma() && spread_mfma(ga, c, fw, s, err)) return err;
-    if (spread_win(ga, c, fw, s, err)) return err;
+    const bool mf = mfma2d_applicable(*ga.g, ga.prec) && use_mfma();
+    if (!mf && spread_win(ga, c, fw, s, err)) return err;  // clears fw itself when asked
+    GridOpArgs gz = ga;
+    gz.zero_out = false;
+    if (ga.zero_out && (err = launch_zero(fw, ga.n_trans * ga.g->grid_cells() * (int64_t)sizeof(cplx<T>), s)) != hipSuccess)
+      return err;
+    if (mf && spread_mfma(gz, c, fw, s, err)) return err;
+    if (mf && spread_win(gz, c, fw, s, err)) return err;
+    switch (ga.g->dim) {
+      case 1: return spread_impl<T, CT, 1>(gz, c, fw, s);
+      case 2: return spread_impl<T, CT, 2>(gz, c, fw, s);
+      case 3: return spread_impl<T, CT, 3>(gz, c, fw, s);
+    }
+    return hipErrorInvalidValue;
+  }
+  if (ga.zero_out) {
+    hipError_t err = launch_zero(fw, ga.n_trans * ga.g->grid_cells() * (int64_t)sizeof(cplx<T>), s);
+    if (err != hipSuccess) return err;
   }
   switch (ga.g->dim) {
     case 1: return spread_impl<T, CT, 1>(ga, c, fw, s);
@@ -239,7 +263,8 @@ hipError_t launch_spread_many(const GridOpArgs* gas, int count, const void* cons
 }
 
 hipError_t launch_spread(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s) {
-  if (ga.ps->M == 0) return hipSuccess;
+  if (ga.ps->M == 0)
+    return ga.zero_out ? launch_zero(fw, ga.n_trans * ga.g->grid_cells() * (ga.prec == kF32 ? 8 : 16), s) : hipSuccess;
   if (ga.prec == kF32) {
     return ga.ps->coord_f64 ? spread_dim<float, double>(ga, c, fw, s) : spread_dim<float, float>(ga, c, fw, s);
   }

@@ -7,7 +7,9 @@ batching of ``nufftax/core/spread.py:100-173`` and the batcher fast path of
 * requests are type-1 or type-2 transforms (1D/2D/3D, complex64/complex128) whose
   inputs already live on the GPU; ``submit`` returns a ``concurrent.futures.Future``;
 * a dispatcher thread admits requests against an HBM budget (points, strengths,
-  fine grid and output of every request in flight) and batches what is queued:
+  fine grid and output of every request in flight) without blocking on one that does not
+  fit while later ones do (those go ahead of it, a bounded number of times), and batches
+  what is queued:
   requests with the same plan signature that share their point set become one
   n_trans unit; with ``max_units`` > 1 and at least ``batch_min_queue`` requests
   queued, up to ``max_units`` units of one kind, dimension, tolerance, sign and precision with
@@ -66,6 +68,7 @@ class ServeRequest:
     t_start: float = 0.0         # taken by the dispatcher
     t_issued: float = 0.0        # all of its launches queued on the stream
     t_done: float = 0.0
+    skipped: int = 0             # times a later request was admitted ahead of this one
     _sig: tuple = None           # signature(), computed once
 
     @property
@@ -116,6 +119,10 @@ class NufftServer:
         # thread, whose issue rate bounds the throughput of small requests
         self.scan_depth = 16
         self.batch_min_queue = 4  # queued requests before point sets share launches
+        # admission does not block on a request that does not fit the budget while later ones
+        # do: those are admitted ahead of it, at most this many times (then it waits first)
+        self.max_skips = 64
+        self.skipped_admissions = 0
         self._in_use = 0
         self._cv = threading.Condition()
         self._q: "queue.Queue[ServeRequest | None]" = queue.Queue()
@@ -182,11 +189,29 @@ class NufftServer:
         self.close()
 
     # ------------------------------------------------------------------ internals
-    def _admit(self, nbytes):
+    def _next_group(self, pending: list):
+        """The next group to issue, its budget already taken: the group led by the oldest
+        pending request if it fits the budget now, else the first group led by a later request
+        (among the next ``scan_depth``) that fits, unless the oldest has been passed over
+        ``max_skips`` times; waits for completions only when nothing fits.  Returns (units, the
+        requests left pending, in arrival order, bytes taken)."""
         with self._cv:
-            while self._in_use + nbytes > self.budget:
+            while True:
+                head = pending[0]
+                last = 1 if head.skipped >= self.max_skips else min(len(pending), self.scan_depth + 1)
+                for start in range(last):
+                    if start and pending[start].nbytes + self._in_use > self.budget:
+                        continue
+                    units, rest = self._take_group(pending[start:])
+                    nbytes = sum(g.nbytes for u in units for g in u)
+                    if self._in_use + nbytes <= self.budget:
+                        self._in_use += nbytes
+                        if start:
+                            self.skipped_admissions += 1
+                            for r in pending[:start]:
+                                r.skipped += 1
+                        return units, pending[:start] + rest, nbytes
                 self._cv.wait()
-            self._in_use += nbytes
 
     def _release(self, nbytes):
         with self._cv:
@@ -251,11 +276,9 @@ class NufftServer:
                 pending = [p for p in pending if p is not None]
             if not pending:
                 break
-            units, pending = self._take_group(pending)
+            units, pending, nbytes = self._next_group(pending)
             r = units[0][0]
             group = [g for u in units for g in u]
-            nbytes = sum(g.nbytes for g in group)
-            self._admit(nbytes)
             if r.stream_index is not None:
                 stream = self.streams[r.stream_index]
             else:

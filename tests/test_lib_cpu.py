@@ -274,3 +274,48 @@ def test_serving_group_formation():
     srv.budget = 250
     units, held = srv._take_group([a, b, c, e])
     assert sum(len(u) for u in units) == 2 and [id(r) for r in held] == [id(c), id(e)]
+
+
+def test_serving_admission_skips_requests_that_do_not_fit():
+    """Admission on host-side requests: a request that does not fit the budget left by the
+    work in flight does not block later ones that fit (they are admitted ahead of it), at
+    most max_skips times; then the oldest request waits for the budget first."""
+    import threading
+
+    import torch
+
+    from nufftax_amd import serving as S
+
+    srv = object.__new__(S.NufftServer)
+    srv.max_group, srv.max_units, srv.batch_max_points, srv.scan_depth, srv.batch_min_queue = 64, 1, 1000, 16, 4
+    srv.budget, srv.max_skips, srv.skipped_admissions, srv._in_use = 1000, 2, 0, 0
+    srv._cv = threading.Condition()
+
+    def req(nbytes, M):
+        pts = (torch.zeros(M), torch.zeros(M))
+        r = S.ServeRequest(1, pts, torch.zeros(M, dtype=torch.complex64), (16, 16), 1e-6, 1)
+        r.nbytes = nbytes
+        return r
+
+    big, s1, s2, s3 = req(800, 10), req(100, 11), req(100, 12), req(100, 13)
+    srv._in_use = 500  # work in flight: big (800) does not fit, the small ones do
+    units, pending, nb = srv._next_group([big, s1, s2, s3])
+    assert units == [[s1]] and nb == 100 and pending == [big, s2, s3] and srv._in_use == 600
+    assert big.skipped == 1 and srv.skipped_admissions == 1
+    units, pending, nb = srv._next_group(pending)
+    assert units == [[s2]] and pending == [big, s3] and big.skipped == 2
+    # big has been passed over max_skips times: it goes first, so admission waits for the
+    # budget that completions release
+    released = []
+
+    def complete():
+        with srv._cv:
+            srv._in_use = 0
+            released.append(True)
+            srv._cv.notify_all()
+
+    t = threading.Timer(0.2, complete)
+    t.start()
+    units, pending, nb = srv._next_group(pending)
+    t.join()
+    assert released and units == [[big]] and pending == [s3] and srv._in_use == 800
