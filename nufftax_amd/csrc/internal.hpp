@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 
 #include "common.hpp"
@@ -87,6 +88,22 @@ struct SortTuning {
 // (window 16 = SB + ns with SB >= 8), square bins of 4 x 4 sub-bins, subproblems of at most
 // 4096 points (eight per thread of a 512-thread block)
 constexpr int kWinMaxSub = 4096;
+// The window spread is the default where it applies; NUFFTX_SPREAD_WIN=0 selects the
+// LDS-atomic bank-spread kernel instead (read when points are set and per call; measured on
+// the 2D radial workload, 4M points, 1024^2 grid: 143 us for packing + the fp16-split window
+// kernel against 200-213 us)
+inline bool win_spread_enabled() {
+  const char* e = std::getenv("NUFFTX_SPREAD_WIN");
+  return e == nullptr || e[0] != '0';
+}
+// NUFFTX_INTERP_WIN=1 turns the window interpolation on (read when points are set and per
+// call).  Off by default: on the 2D radial workload it took 113-126 us against 73 us for the
+// wide gather (its 16-point groups run a long dependent chain per wave: weights, LDS rows,
+// split, three MFMAs, dot product, two cross-lane adds).
+inline bool win_interp_enabled() {
+  const char* e = std::getenv("NUFFTX_INTERP_WIN");
+  return e != nullptr && e[0] == '1';
+}
 inline bool win2d_geometry(const Geometry& g, int prec, int maxsub) {
   if (g.dim != 2 || prec != 0 || g.ns > 8 || maxsub > kWinMaxSub) return false;
   const int bs = 4 * (16 - g.ns);

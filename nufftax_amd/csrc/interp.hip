@@ -3,6 +3,7 @@
 #include <cstdlib>
 
 #include "si_mfma.hpp"
+#include "si_win.hpp"
 
 namespace nufftx {
 
@@ -76,11 +77,39 @@ bool interp_mfma(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s, h
   return false;
 }
 
+// The matrix-core window interpolation (2D FP32, ns <= 8, window bins, compact records).
+bool interp_win(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s, hipError_t& err) {
+  const PointSet& ps = *ga.ps;
+  if (!win2d_geometry(*ga.g, ga.prec, ps.maxsub) || !ps.has_crec || !win_interp_enabled()) return false;
+  const SIArgs<float> a = make_si_args<float>(ga);
+  if (!a.crec) return false;
+  auto launch = [&](auto nsc) -> bool {
+    constexpr int NS = decltype(nsc)::value;
+    const size_t lds = wi_interp_lds<NS>(ps.maxsub);
+    // static LDS (ordering counters) below 2.2 KB
+    if (lds + 2304 > (size_t)lds_device_max_bytes()) return false;
+    auto kern = k_interp_win2dh<NS>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(kWiThreads), lds, s, a, ps.maxsub,
+                       static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));
+    err = hipGetLastError();
+    return true;
+  };
+  switch (ga.g->ns) {
+    case 2: return launch(std::integral_constant<int, 2>{});
+    case 4: return launch(std::integral_constant<int, 4>{});
+    case 6: return launch(std::integral_constant<int, 6>{});
+    case 8: return launch(std::integral_constant<int, 8>{});
+  }
+  return false;
+}
+
 template <typename T, typename CT>
 hipError_t interp_dim(const GridOpArgs& ga, const void* fw, void* c, hipStream_t s) {
   if constexpr (std::is_same<T, float>::value) {
     hipError_t err = hipSuccess;
     if (mfma2d_applicable(*ga.g, ga.prec) && use_mfma() && interp_mfma(ga, fw, c, s, err)) return err;
+    if (interp_win(ga, fw, c, s, err)) return err;
   }
   switch (ga.g->dim) {
     case 1: return interp_impl<T, CT, 1>(ga, fw, c, s);

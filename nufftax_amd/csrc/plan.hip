@@ -391,9 +391,11 @@ int check_plan(nufftx_plan p) {
 int sort_points(nufftx_plan p, PointSet& ps, const Geometry& g, bool interp, bool spread) {
   std::string err;
   SortTuning tune = p->tune;
-  // point sets whose forward use is the window spread get subproblems of up to 4096 points
-  // (one block of the window kernel per bin of the 2D radial workload)
-  if (spread && win2d_geometry(g, p->prec, kWinMaxSub) && !std::getenv("NUFFTX_MAXSUB")) tune.maxsub = kWinMaxSub;
+  // point sets that the window kernels spread or interpolate get subproblems of up to 4096
+  // points (one block of the window kernels per bin of the 2D radial workload)
+  if (((spread && win_spread_enabled()) || (interp && win_interp_enabled())) && win2d_geometry(g, p->prec, kWinMaxSub) &&
+      !std::getenv("NUFFTX_MAXSUB"))
+    tune.maxsub = kWinMaxSub;
   const bool subsort = subsort_applicable(tune, g, p->prec);
   // records pay for their cost in the sort only where the point set is interpolated;
   // the matrix-core kernels need the full records with sub-bins

@@ -633,6 +633,252 @@ __global__ void __launch_bounds__(kWhThreads) k_spread_win2dh(SIArgs<float> a, i
   }
 }
 
+// ---- fp16-split window interpolation (2D FP32, the transpose of the window spread)
+//
+// The values of the points of one sub-bin read one 16 x 16 window F of the tile:
+//   v_j = sum_y wy_j(y) sum_x F[y][x] wx_j(x)                (nufftax/core/spread.py:596-658)
+// so for a group of 16 points G = F Bx with Bx[x][j] = wx_j(x) is one 16 x 16 x 16 matrix
+// product (v_mfma_f32_16x16x16_f16, three per part with fp16-split operands as in the
+// spread), and v_j = sum_y wy_j(y) G[y][j] is a 16-term dot product: each lane holds four y
+// of one point, two cross-lane adds finish it.  The tile is loaded once per subproblem; a
+// window is split into fp16 parts once per sub-bin a wave visits.  Scaling: A = ldexp(F,
+// 15 - e) with 2^e above the tile's max |re|, |im|, B = 2^14 wx.  A non-finite tile value
+// makes the values of the points of its window non-finite (the product meets it with a zero
+// weight).  Points come from the compact records (sorted order, perm gives the output index).
+constexpr int kWiThreads = 512;                  // 8 waves per subproblem
+constexpr int kWiPpt = 8;                        // points per thread while ordering (<= 4096)
+constexpr int kWiK = 16;                         // points per group (K of the f16 MFMA's output columns)
+constexpr int kWiRowF = kWiK * kWinRow + 16;     // floats of one row block (+ trailing zeros)
+constexpr int kWiWaveF = 2 * kWiRowF + 4 * kWiK; // floats per wave: x rows, y rows, meta
+static_assert(kWiThreads * kWiPpt == kWinMaxSub, "the interpolation orders subproblems of up to kWinMaxSub points");
+
+typedef _Float16 wi_h4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline size_t wi_order_bytes(int maxsub) {
+  return ((size_t)(maxsub + 16 * (kWiK - 1)) * sizeof(uint16_t) + 15) / 16 * 16;
+}
+
+// LDS bytes of the window interpolation (dynamic part) for subproblems of maxsub points
+template <int NS>
+inline size_t wi_interp_lds(int maxsub) {
+  constexpr int T = WinGeom<NS>::T;
+  return (size_t)T * T * sizeof(cplx<float>) + wi_order_bytes(maxsub) + (size_t)(kWiThreads / 64) * kWiWaveF * sizeof(float);
+}
+
+__device__ __forceinline__ void split4_f16(const float (&v)[4], wi_h4& hi, wi_h4& lo) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) hi[e] = (_Float16)v[e];
+  asm("" : "+v"(hi));  // one hi for both uses (see split_f16)
+#pragma unroll
+  for (int e = 0; e < 4; ++e) lo[e] = (_Float16)(v[e] - (float)hi[e]);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kWiThreads) k_interp_win2dh(SIArgs<float> a, int maxsub,
+                                                              const cplx<float>* __restrict__ fw,
+                                                              cplx<float>* __restrict__ out) {
+  using G = WinGeom<NS>;
+  constexpr int SB = G::SB, T = G::T, TC = T * T;
+  constexpr int W = kWiThreads / 64;
+  constexpr uint16_t kPad = 0xFFFF;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<float>* tile = reinterpret_cast<cplx<float>*>(smem_raw);
+  uint16_t* order = reinterpret_cast<uint16_t*>(tile + TC);
+  __shared__ unsigned short wcnt[kWiPpt * W * 16];
+  __shared__ int cnt[16], start[17];
+  __shared__ unsigned smax;
+  const int sub = blockIdx.x;
+  if (sub >= *a.nsub) return;
+  const int tr = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int4 sd = a.subprob[sub];
+  const int n = sd.z;
+  const uint2* __restrict__ rec = a.crec + sd.y;
+  float* const lf = reinterpret_cast<float*>(smem_raw);
+  const int wof = (int)(((size_t)TC * sizeof(cplx<float>) + wi_order_bytes(maxsub)) / 4) + wave * kWiWaveF;
+  float* rowx = lf + wof;
+  float* rowy = rowx + kWiRowF;
+  int4* meta = reinterpret_cast<int4*>(rowy + kWiRowF);  // {y read index, x read index, output index, -}
+  int org[3];
+  tile_origin<2>(sd, a.bs, a.nb, NS / 2, org);
+  const int tile3[3] = {T, T, 1};
+  const cplx<float>* fT = fw + (int64_t)tr * a.grid_cells;
+  if (tid == 0) smax = 0u;
+  for (int e = tid; e < kWiPpt * W * 16; e += kWiThreads) wcnt[e] = 0;
+  for (int e = lane; e < kWiRowF; e += 64)
+    if (e % kWinRow < 16 - NS || e % kWinRow == 16 || e >= kWiK * kWinRow) {
+      rowx[e] = 0.0f;
+      rowy[e] = 0.0f;
+    }
+  __syncthreads();
+  // ---- 1. tile and its max |re|, |im|
+  unsigned mymax = 0;
+  for (int cell = tid; cell < TC; cell += kWiThreads) {
+    const cplx<float> v = fT[tile_cell_to_global<2>(cell, tile3, org, a.nf)];
+    tile[cell] = v;
+    mymax = max(mymax, max(__float_as_uint(fabsf(v.re)), __float_as_uint(fabsf(v.im))));
+  }
+  // ---- order the points by sub-bin (as in the window spread, groups of 16)
+  int sb[kWiPpt];
+  const int npass = (n + kWiThreads - 1) / kWiThreads;  // passes that hold points (uniform)
+#pragma unroll
+  for (int r = 0; r < kWiPpt; ++r) {
+    const int j = tid + r * kWiThreads;
+    sb[r] = -1;
+    if (j < n) {
+      const uint2 p = rec[j];
+      sb[r] = min((int)(p.y >> 25) / SB, 3) * 4 + min((int)(p.x >> 25) / SB, 3);
+    }
+  }
+  int rank[kWiPpt];
+#pragma unroll
+  for (int r = 0; r < kWiPpt; ++r) {
+    if (r >= npass) break;
+    const bool valid = sb[r] >= 0;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const uint64_t bal = __ballot(valid && ((sb[r] >> b) & 1));
+      m &= ((sb[r] >> b) & 1) ? bal : ~bal;
+    }
+    rank[r] = popc_below(m);
+    if (valid && rank[r] == 0) wcnt[(r * W + wave) * 16 + sb[r]] = (unsigned short)__popcll(m);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mymax = max(mymax, (unsigned)__shfl_xor((int)mymax, off, 64));
+  if (lane == 0) atomicMax(&smax, mymax);
+  __syncthreads();
+  if (tid < 16) {
+    int c = 0;
+    for (int e = 0; e < kWiPpt * W; ++e) c += wcnt[e * 16 + tid];
+    cnt[tid] = c;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int s = 0; s < 16; ++s) {
+      start[s] = acc;
+      acc += (cnt[s] + kWiK - 1) & ~(kWiK - 1);
+    }
+    start[16] = acc;
+  }
+  __syncthreads();
+  if (tid < 16) {
+    int run = start[tid];
+    for (int e = 0; e < kWiPpt * W; ++e) {
+      const int v = wcnt[e * 16 + tid];
+      wcnt[e * 16 + tid] = (unsigned short)run;
+      run += v;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kWiPpt; ++r)
+    if (r < npass && sb[r] >= 0) order[wcnt[(r * W + wave) * 16 + sb[r]] + rank[r]] = (uint16_t)(tid + r * kWiThreads);
+  if (tid < 16 * (kWiK - 1)) {
+    const int s = tid / (kWiK - 1), q = tid % (kWiK - 1);
+    const int slot = start[s] + cnt[s] + q;
+    if (slot < start[s + 1]) order[slot] = kPad;
+  }
+  const float fm = __uint_as_float(smax);
+  int ke = 0;  // A = ldexp(F, ke), |A| < 2^15
+  if (isfinite(fm) && fm > 0.0f) {
+    int ef;
+    (void)frexpf(fm, &ef);
+    ke = 15 - ef;
+  }
+  const int kout = -ke - 14;  // values come out in units of 2^(ke + 14)
+  __syncthreads();
+
+  // ---- 2. groups of 16 points of one sub-bin, contiguous ranges per wave
+  const int NG = start[16] / kWiK;
+  const int g0 = (int)(((int64_t)NG * wave) / W), g1 = (int)(((int64_t)NG * (wave + 1)) / W);
+  const int i = lane & 15, kq = lane >> 4;
+  // weight staging: lane -> (point sq, dimension sdim, weights sh NS/2 .. sh NS/2 + NS/2 - 1)
+  const int sq = lane & (kWiK - 1), sdim = (lane >> 4) & 1, sh = lane >> 5;
+  const float bl2e = a.es.beta_log2e, kc = a.es.c;
+  float* const myrow = (sdim ? rowy : rowx) + kWinRow * sq + (16 - NS) + sh * (NS / 2);
+  const int ry0 = wof + kWiRowF + (16 - NS), rx0 = wof + (16 - NS);
+  int s = -1;  // sub-bin whose window is split in registers
+  int next = 0;
+  wi_h4 fre_h, fre_l, fim_h, fim_l;
+  // the next group's record and output index load while the current group runs
+  auto fetch = [&](int g, uint2& pr, int& outi) {
+    const int j = g < g1 ? (int)order[kWiK * g + sq] : (int)kPad;
+    pr = j != kPad ? rec[j] : make_uint2(0u, 0u);
+    outi = j != kPad ? a.perm[sd.y + j] : -1;
+  };
+  uint2 p;
+  int outi;
+  fetch(g0, p, outi);
+  for (int g = g0; g < g1; ++g) {
+    if (kWiK * g >= next) {  // a new sub-bin: its window, split into fp16 parts
+      do {
+        ++s;
+      } while (s < 15 && start[s + 1] <= kWiK * g);
+      next = start[s + 1];
+      // A[i][k] = F[y = i][x = 4 kq + e]: four consecutive cells of tile row sy SB + i
+      const cplx<float>* row = tile + ((s >> 2) * SB + i) * T + (s & 3) * SB + 4 * kq;
+      float vr[4], vi[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const cplx<float> v = row[e];
+        vr[e] = ldexpf(v.re, ke);
+        vi[e] = ldexpf(v.im, ke);
+      }
+      split4_f16(vr, fre_h, fre_l);
+      split4_f16(vi, fim_h, fim_l);
+    }
+    {
+      const unsigned wd = sdim ? p.y : p.x;
+      const float t = (float)((double)(wd & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1));
+      const float wsc = sdim ? 1.0f : 16384.0f;  // B = 2^14 wx
+#pragma unroll
+      for (int m = 0; m < NS / 2; ++m) myrow[m] = es_phi_in(t + (float)(sh * (NS / 2) + m), bl2e, kc) * wsc;
+      if (lane < kWiK) {
+        const int lx = (int)(p.x >> 25), ly = (int)(p.y >> 25);
+        const int ox = lx - min(lx / SB, 3) * SB, oy = ly - min(ly / SB, 3) * SB;
+        meta[sq] = make_int4(ry0 + kWinRow * sq - oy, rx0 + kWinRow * sq - ox, outi, 0);
+      }
+    }
+    fetch(g + 1, p, outi);
+    wave_sync_win();
+    // B[k = x][j] in lane j + 16 (x / 4): point i, x = 4 kq + e; wy for y = 4 kq + e likewise
+    const int4 mt = meta[i];
+    float bx[4], wy[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      bx[e] = lf[mt.y + 4 * kq + e];
+      wy[e] = lf[mt.x + 4 * kq + e];
+    }
+    wave_sync_win();
+    wi_h4 bh, bl;
+    split4_f16(bx, bh, bl);
+    const win_f4 z = {0.f, 0.f, 0.f, 0.f};
+    win_f4 gre = __builtin_amdgcn_mfma_f32_16x16x16f16(fre_l, bh, z, 0, 0, 0);
+    gre = __builtin_amdgcn_mfma_f32_16x16x16f16(fre_h, bl, gre, 0, 0, 0);
+    gre = __builtin_amdgcn_mfma_f32_16x16x16f16(fre_h, bh, gre, 0, 0, 0);
+    win_f4 gim = __builtin_amdgcn_mfma_f32_16x16x16f16(fim_l, bh, z, 0, 0, 0);
+    gim = __builtin_amdgcn_mfma_f32_16x16x16f16(fim_h, bl, gim, 0, 0, 0);
+    gim = __builtin_amdgcn_mfma_f32_16x16x16f16(fim_h, bh, gim, 0, 0, 0);
+    // G[y = 4 kq + e][point i] in element e: dot with wy, then over the four lane groups
+    float sr = 0.0f, si = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      sr = fmaf(gre[e], wy[e], sr);
+      si = fmaf(gim[e], wy[e], si);
+    }
+    sr += __shfl_xor(sr, 16, 64);
+    si += __shfl_xor(si, 16, 64);
+    sr += __shfl_xor(sr, 32, 64);
+    si += __shfl_xor(si, 32, 64);
+    if (lane < kWiK && mt.z >= 0 && mt.z < a.M)
+      out[(int64_t)tr * a.M + mt.z] = cplx<float>{ldexpf(sr, kout), ldexpf(si, kout)};
+  }
+}
+
+
 // NUFFTX_WIN_F32=1: the FP32 window kernel (v_mfma_f32_16x16x4f32, subproblems of at most
 // 1024 points) instead of the fp16-split one
 inline bool win_f32_kernel() {
@@ -646,13 +892,5 @@ inline bool win_exact_tiles() {
   return e != nullptr && e[0] == '1';
 }
 
-// The window spread is the default where it applies; NUFFTX_SPREAD_WIN=0 selects the
-// LDS-atomic bank-spread kernel instead (measured on the 2D radial workload, 4M points,
-// 1024^2 grid: 142 us for the window kernel + 41 us for packing the points, against
-// 199-213 us).
-inline bool use_win_spread() {
-  const char* e = std::getenv("NUFFTX_SPREAD_WIN");
-  return e == nullptr || e[0] != '0';
-}
 
 }  // namespace nufftx

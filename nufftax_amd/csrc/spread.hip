@@ -114,7 +114,7 @@ bool spread_mfma(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, h
 // spread from those contiguous records.
 bool spread_win(const GridOpArgs& ga, const void* c, void* fw, hipStream_t s, hipError_t& err) {
   PointSet& ps = const_cast<PointSet&>(*ga.ps);
-  if (!win2d_geometry(*ga.g, ga.prec, ps.maxsub) || !ps.has_slot || !use_win_spread()) return false;
+  if (!win2d_geometry(*ga.g, ga.prec, ps.maxsub) || !ps.has_slot || !win_spread_enabled()) return false;
   SIArgs<float> a = make_si_args<float>(ga);
   // zero_out: the packing pass clears the grid (n_trans grids of complex64; an odd cell count
   // leaves 8 bytes over the pass's 16-byte stores, and takes the fill kernel)

@@ -458,12 +458,20 @@ def test_type3_small_grids_vs_direct(rng, dim, prec):
 
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
-@pytest.mark.parametrize("mfma", ["1", None])
+@pytest.mark.parametrize("mfma", ["1", None, "f32win", "bank"])
 def test_spread2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
-    """2D FP32 spreading (LDS-tiled, and the matrix-core path for ns <= 8) vs the FP64 oracle, every even ns."""
+    """2D FP32 spreading vs the FP64 oracle, every even ns: the default (fp16-split window kernel
+    for ns <= 8, 4096-point subproblems), the FP32 window kernel (NUFFTX_WIN_F32 with
+    1024-point subproblems), the bank-spread kernel (NUFFTX_SPREAD_WIN=0) and the records-based
+    matrix-core path (NUFFTX_MFMA)."""
     from nufftax_amd._plan import F32, grid_plan
 
-    if mfma:
+    if mfma == "f32win":
+        monkeypatch.setenv("NUFFTX_WIN_F32", "1")
+        monkeypatch.setenv("NUFFTX_MAXSUB", "1024")
+    elif mfma == "bank":
+        monkeypatch.setenv("NUFFTX_SPREAD_WIN", "0")
+    elif mfma:
         monkeypatch.setenv("NUFFTX_MFMA", mfma)
     kp = R.compute_kernel_params(eps)
     nfs = (4 * (16 - kp.nspread) * 3 + 5, 4 * (16 - kp.nspread) * 2 + 3)  # several bins, ragged edge
@@ -484,13 +492,16 @@ def test_spread2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
 
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
-@pytest.mark.parametrize("mfma", ["1", None])
+@pytest.mark.parametrize("mfma", ["1", None, "win"])
 def test_interp2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
-    """2D FP32 interpolation (LDS-tiled, and the matrix-core kernel for ns <= 8) vs the FP64 oracle, every even
-    ns, batched."""
+    """2D FP32 interpolation (LDS-tiled; the records-based matrix-core kernel, NUFFTX_MFMA; the
+    fp16-split window interpolation, NUFFTX_INTERP_WIN, for ns <= 8) vs the FP64 oracle, every
+    even ns, batched."""
     from nufftax_amd._plan import F32, grid_plan
 
-    if mfma:
+    if mfma == "win":
+        monkeypatch.setenv("NUFFTX_INTERP_WIN", "1")
+    elif mfma:
         monkeypatch.setenv("NUFFTX_MFMA", mfma)
     kp = R.compute_kernel_params(eps)
     nfs = (4 * (16 - kp.nspread) * 3 + 5, 4 * (16 - kp.nspread) * 2 + 3)
