@@ -71,8 +71,9 @@ struct PointSet {
   // words as in crec), so that the spread reads its points contiguously
   int32_t* slot = nullptr;       // M
   bool has_slot = false;
-  uint4* packed = nullptr;       // n_trans * M
-  int64_t cap_slot = 0, cap_packed = 0;
+  uint4* packed = nullptr;       // n_trans * M (2D); M {x, y, z word, 0} (3D)
+  float2* cs = nullptr;          // 3D: the strengths of every transform in sorted order, n_trans * M
+  int64_t cap_slot = 0, cap_packed = 0, cap_cs = 0;
   int64_t cap_recs = 0, cap_sub_off = 0, cap_cmax = 0;
   int64_t cap_M = 0, cap_keys = 0, cap_hist = 0, cap_part = 0, cap_bins = 0, cap_sub = 0, cap_sub_start = 0;
   std::vector<void*> retired;    // outgrown buffers, freed by pointset_free
@@ -88,6 +89,20 @@ struct SortTuning {
 // (window 16 = SB + ns with SB >= 8), square bins of 4 x 4 sub-bins, subproblems of at most
 // 4096 points (eight per thread of a 512-thread block)
 constexpr int kWinMaxSub = 4096;
+// Whether a geometry takes the 3D matrix-core window spread (si_win3.hpp): 3D FP32, ns = 8
+// (its windows cost the same at every ns, and the bank-spread kernel's ns^3 atomics per point
+// are cheaper below 8: 1.06 against 1.48 ms for 8M clustered points at ns = 6), bins of whole
+// 8 x 8 x 4 sub-bins (at most 16 of them), tiles of at most 127 cells a side, subproblems of
+// at most 2048 points (four per thread of a 512-thread block)
+constexpr int kWin3MaxSub = 2048;
+inline bool win3d_geometry(const Geometry& g, int prec, int maxsub) {
+  if (g.dim != 3 || prec != 0 || g.ns != 8 || maxsub > kWin3MaxSub) return false;
+  if (g.bs[0] % 8 || g.bs[1] % 8 || g.bs[2] % 4) return false;
+  if ((g.bs[0] / 8) * (g.bs[1] / 8) * (g.bs[2] / 4) > 16) return false;
+  for (int d = 0; d < 3; ++d)
+    if (g.tile[d] > 127 || g.tile[d] != g.bs[d] + g.ns) return false;
+  return true;
+}
 // The window spread is the default where it applies; NUFFTX_SPREAD_WIN=0 selects the
 // LDS-atomic bank-spread kernel instead (read when points are set and per call; measured on
 // the 2D radial workload, 4M points, 1024^2 grid: 143 us for packing + the fp16-split window
@@ -129,14 +144,18 @@ struct SortSizes {
 };
 SortSizes sort_sizes(int64_t M, const Geometry& g, const SortTuning& tune, int rec_kind);
 
-// want_slot (2D): also write the inverse permutation, and size the packed records for n_trans
-// transforms (n_trans 0: left to the first spread)
+// want_slot (2D, 3D): also write the inverse permutation, and size the packed records for
+// n_trans transforms (n_trans 0: left to the first spread)
 hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune, hipStream_t s, std::string& err,
                          int rec_kind = 0, int sub_sb = 0, bool want_slot = false, int64_t n_trans = 1);
 // packed[t][slot[i]] = {x word, y word, c[t][i]} of a 2D point set sorted with want_slot (FP32
 // strengths; a non-finite coordinate makes its strength NaN); the same launch clears
 // zero_bytes (a multiple of 16) at zero, when given
 hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s,
+                           void* zero = nullptr, int64_t zero_bytes = 0);
+// 3D: packed[slot[i]] = {x word, y word, z word, 0}, cs[t][slot[i]] = c[t][i] (NaN for a
+// non-finite coordinate); the same launch clears zero_bytes (a multiple of 16) at zero
+hipError_t pointset_pack3d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s,
                            void* zero = nullptr, int64_t zero_bytes = 0);
 
 // Device buffer of at least `need` elements (contents not kept when it grows).  The

@@ -506,6 +506,48 @@ __global__ void __launch_bounds__(256) k_pack2d(RecArgs ra, const int32_t* __res
   }
 }
 
+// Packed words of a 3D point set for the 3D window spread: packed[slot[i]] = {x, y, z word}
+// (the layout of compact_word, tile-relative to the point's bin from its stored key) and
+// cs[t][slot[i]] = c[t][i], NaN for a non-finite coordinate.  Index-order reads, scattered
+// writes; the first loop clears the spread's output grid when asked.
+struct Pack3Args {
+  const void* coord[3];
+  int nf[3], bs[3], nb[3], ns;
+  const int32_t* slot;
+};
+
+template <typename CT>
+__global__ void __launch_bounds__(256) k_pack3d(Pack3Args pa, const int32_t* __restrict__ keys, int64_t M,
+                                                int64_t n_trans, const float2* __restrict__ c,
+                                                uint4* __restrict__ packed, float2* __restrict__ cs,
+                                                uint4* __restrict__ zero, int64_t nzero) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nzero; i += stride)
+    zero[i] = make_uint4(0u, 0u, 0u, 0u);
+  const int half = pa.ns / 2;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride) {
+    const int pos = pa.slot[i];
+    if ((unsigned)pos >= (unsigned)M) continue;
+    int key = keys[i];
+    unsigned w[3];
+    bool fin = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int b = key % pa.nb[d];
+      key /= pa.nb[d];
+      const double x = (double)static_cast<const CT*>(pa.coord[d])[i];
+      fin = fin && isfinite(x);
+      w[d] = compact_word(x, pa.nf[d], b * pa.bs[d], half);
+    }
+    packed[pos] = make_uint4(w[0], w[1], w[2], 0u);
+    for (int64_t t = 0; t < n_trans; ++t) {
+      float2 v = c[t * M + i];
+      if (!fin) v = make_float2(NAN, NAN);
+      cs[t * M + pos] = v;
+    }
+  }
+}
+
 __global__ void k_zero_i32(int32_t* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -648,10 +690,16 @@ hipError_t pointset_sort(PointSet& ps, const Geometry& g, const SortTuning& tune
   if (recs && (e = grow_buffer(ps.recs, ps.cap_recs, M, ps.retired)) != hipSuccess) return e;
   if (crec && (e = grow_buffer(ps.crec, ps.cap_crec, M, ps.retired)) != hipSuccess) return e;
   // the inverse permutation also serves the compact records (written by a pass of their own)
-  const bool slot = (want_slot || crec) && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
+  const bool slot2 = (want_slot || crec) && g.dim == 2 && M > 0 && g.bs[0] <= 127 && g.bs[1] <= 127;
+  const bool slot3 = want_slot && g.dim == 3 && M > 0 && g.tile[0] <= 127 && g.tile[1] <= 127 && g.tile[2] <= 127;
+  const bool slot = slot2 || slot3;
   if (slot && (e = grow_buffer(ps.slot, ps.cap_slot, M, ps.retired)) != hipSuccess) return e;
-  if (slot && n_trans > 0 &&
+  if (slot2 && n_trans > 0 &&
       (e = grow_buffer(ps.packed, ps.cap_packed, M * n_trans, ps.retired)) != hipSuccess)
+    return e;
+  if (slot3 && n_trans > 0 &&
+      ((e = grow_buffer(ps.packed, ps.cap_packed, M, ps.retired)) != hipSuccess ||
+       (e = grow_buffer(ps.cs, ps.cap_cs, M * n_trans, ps.retired)) != hipSuccess))
     return e;
   ps.has_recs = false;
   ps.has_crec = false;
@@ -742,9 +790,36 @@ hipError_t pointset_pack2d(PointSet& ps, const Geometry& g, const void* c, int64
   return hipGetLastError();
 }
 
+hipError_t pointset_pack3d(PointSet& ps, const Geometry& g, const void* c, int64_t n_trans, hipStream_t s,
+                           void* zero, int64_t zero_bytes) {
+  if (!ps.has_slot || g.dim != 3 || zero_bytes % 16 != 0) return hipErrorInvalidValue;
+  if (ps.M == 0) return hipSuccess;
+  hipError_t e;
+  if ((e = grow_buffer(ps.packed, ps.cap_packed, ps.M, ps.retired)) != hipSuccess) return e;
+  if ((e = grow_buffer(ps.cs, ps.cap_cs, ps.M * n_trans, ps.retired)) != hipSuccess) return e;
+  Pack3Args pa{};
+  for (int d = 0; d < 3; ++d) {
+    pa.coord[d] = ps.coord[d];
+    pa.nf[d] = g.nf[d];
+    pa.bs[d] = g.bs[d];
+    pa.nb[d] = g.nb[d];
+  }
+  pa.ns = g.ns;
+  pa.slot = ps.slot;
+  const int64_t nzero = zero ? zero_bytes / 16 : 0;
+  const unsigned nb = (unsigned)std::min<int64_t>((std::max<int64_t>(ps.M, nzero) + 255) / 256, 8192);
+  if (ps.coord_f64)
+    hipLaunchKernelGGL(k_pack3d<double>, dim3(nb), dim3(256), 0, s, pa, ps.keys, ps.M, n_trans,
+                       static_cast<const float2*>(c), ps.packed, ps.cs, static_cast<uint4*>(zero), nzero);
+  else
+    hipLaunchKernelGGL(k_pack3d<float>, dim3(nb), dim3(256), 0, s, pa, ps.keys, ps.M, n_trans,
+                       static_cast<const float2*>(c), ps.packed, ps.cs, static_cast<uint4*>(zero), nzero);
+  return hipGetLastError();
+}
+
 void pointset_free(PointSet& ps) {
   void* bufs[] = {ps.perm, ps.keys, ps.hist, ps.scan_part, ps.scan_sum, ps.bin_start, ps.sub_start, ps.subprob,
-                  ps.nsub, ps.recs, ps.crec, ps.sub_off, ps.cmax, ps.slot, ps.packed};
+                  ps.nsub, ps.recs, ps.crec, ps.sub_off, ps.cmax, ps.slot, ps.packed, ps.cs};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   for (void* p : ps.retired) (void)hipFree(p);
@@ -759,8 +834,9 @@ void pointset_free(PointSet& ps) {
   ps.cmax = nullptr;
   ps.slot = nullptr;
   ps.packed = nullptr;
+  ps.cs = nullptr;
   ps.has_slot = false;
-  ps.cap_slot = ps.cap_packed = 0;
+  ps.cap_slot = ps.cap_packed = ps.cap_cs = 0;
   ps.has_recs = false;
   ps.recs_subbin = false;
   ps.cap_recs = ps.cap_sub_off = ps.cap_cmax = 0;

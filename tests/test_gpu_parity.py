@@ -520,6 +520,40 @@ def test_interp2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
     assert R.rel_l2(_np(c), ref) < 2e-6
 
 
+@pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
+@pytest.mark.parametrize("dist", ["uniform", "clustered"])
+@pytest.mark.parametrize("path", ["window", "packed", "bank"])
+def test_spread3d_f32_every_nspread(rng, eps, dist, path, monkeypatch):
+    """3D FP32 spreading vs the FP64 oracle, two transforms, rel l2 < 2e-6: the default paths
+    (the matrix-core window kernel with exact tiles at ns = 8, eps 1e-6: bins of 16 x 8 x 8
+    cells, 8 x 8 x 4 sub-bins, ragged last bins; the bank-spread kernel below ns = 8), the
+    window kernel with packed tiles (NUFFTX_WIN3_PACKED=1) and the bank-spread kernel at every
+    ns (NUFFTX_SPREAD_WIN=0)."""
+    from nufftax_amd._plan import F32, grid_plan
+
+    if path == "bank":
+        monkeypatch.setenv("NUFFTX_SPREAD_WIN", "0")
+    if path == "packed":
+        if eps != 1e-6:
+            pytest.skip("the window kernel takes ns = 8 only")
+        monkeypatch.setenv("NUFFTX_WIN3_PACKED", "1")
+    kp = R.compute_kernel_params(eps)
+    nfs = (44, 40, 28)
+    M = 20000
+    if dist == "uniform":
+        coords = [rng.uniform(-np.pi, np.pi, M) for _ in range(3)]
+    else:
+        coords = [np.concatenate([rng.normal(-0.5, 0.08, M - 3000), rng.uniform(-np.pi, np.pi, 3000)]) for _ in range(3)]
+    coords = [x.astype(np.float32) for x in coords]
+    c = (rng.standard_normal((2, M)) + 1j * rng.standard_normal((2, M))).astype(np.complex64)
+    plan = grid_plan(3, nfs, kp.nspread, kp.beta, kp.c, 2, F32, torch.device(DEV))
+    plan.setpts([_t(x) for x in coords])
+    fw = torch.full((2,) + tuple(reversed(nfs)), float("nan"), dtype=torch.complex64, device=DEV)
+    plan.spread(_t(c), fw)
+    ref = R.spread([x.astype(np.float64) for x in coords], c.astype(np.complex128), nfs, kp)
+    assert R.rel_l2(_np(fw), ref) < 2e-6
+
+
 @pytest.mark.parametrize("eps", [1e-2, 1e-4, 1e-6])
 @pytest.mark.parametrize("wide", ["1", "0"])
 def test_interp3d_f32_wide_reads(rng, eps, wide, monkeypatch):
