@@ -75,8 +75,8 @@ __global__ void __launch_bounds__(kW3Threads) k_spread_win3dh(SIArgs<float> a, i
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int T0 = a.tile[0], T1 = a.tile[1], T2 = a.tile[2];
   const int TC = T0 * T1 * T2;
+  // sub-bins per bin: nsx nsy nsz <= 16 (win3d_geometry)
   const int nsx = a.bs[0] / kW3SB, nsy = a.bs[1] / kW3SB, nsz = a.bs[2] / kW3SBZ;
-  const int nsb = nsx * nsy * nsz;  // <= 16
   Word* tile = reinterpret_cast<Word*>(smem_raw);  // packed words, or real parts
   Word* tim = tile + TC;                            // imaginary parts (exact tiles)
   uint16_t* order = reinterpret_cast<uint16_t*>(tile + (PK ? 1 : 2) * TC);
