@@ -455,7 +455,9 @@ def main():
         # pack: coordinates, key, slot, strength read (4 + 4 + 4 + 4 + 8 B), 16-B record written;
         # window spread: the 16-B records read, the grid written once
         f32win = os.environ.get("NUFFTX_WIN_F32", "0") == "1"
-        kname = "k_pack2d<float> + " + ("k_spread_win2d<8, true>" if f32win else "k_spread_win2dh<8>")
+        poly = os.environ.get("NUFFTX_ES_POLY", "1") != "0"  # polynomial kernel weights (ns = 8)
+        kname = "k_pack2d<float> + " + ("k_spread_win2d<8, true>" if f32win else
+                                        "k_spread_win2dh<8, %s>" % ("true" if poly else "false"))
         bytes_spread = M * (24 + 16) + M * 16 + 8 * nfc
     else:
         kname = "k_absmax + k_spread_banked<float, 2, 8, false, 16, false>"

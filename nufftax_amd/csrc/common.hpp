@@ -89,6 +89,51 @@ __device__ __forceinline__ float es_phi_in(float z, float beta_log2e, float c) {
   return __builtin_amdgcn_exp2f(-beta_log2e * cz2 * __builtin_amdgcn_rcpf(1.0f + s));
 }
 
+// ---------------------------------------------------------------------------
+// Polynomial ES kernel for ns = 8 (FP32 kernels).  The weight of support interval k of a point
+// (z = t + k, t = i0 - xs in [-ns/2, 1 - ns/2]) is phi(z) = P_k(v), v = 2 (t + ns/2) - 1 in
+// [-1, 1], one v for all k: the piecewise-polynomial kernel evaluation of FINUFFT (Barnett,
+// Magland, af Klinteberg 2019).  phi is even, so P_{ns-1-k}(v) = P_k(-v), and with
+// P_k(v) = E_k(v^2) + v O_k(v^2) (degree 9: E_k and O_k of degree 4) a pair of weights costs
+// 10 FMAs: w[k] = E + v O, w[ns-1-k] = E - v O; 40 FP32 coefficients in all, against 7 plain
+// and 3 transcendental instructions (sqrt, rcp, exp2) per weight for es_phi_in.  The host
+// interpolates P_k at Chebyshev points in FP64 (es_poly_table), evaluates the FP32 form as the
+// kernels do at 257 points per interval against the exact phi, and switches it on only where
+// the largest error is at most kEsPolyTol (phi(0) = 1): at upsampfac 2, 8e-8 (the FP32
+// rounding of values near 1) on the inner intervals and 5e-9 on the two outer ones, where the
+// square root's branch point at |z| = ns/2 stops the convergence near exp(-beta).
+// ---------------------------------------------------------------------------
+constexpr int kEsPolyNs = 8;
+constexpr double kEsPolyTol = 2.5e-7;
+struct EsPoly {
+  float e[kEsPolyNs / 2][5];  // E_k(u) = sum_d e[k][d] u^d
+  float o[kEsPolyNs / 2][5];  // O_k(u) = sum_d o[k][d] u^d
+  int on;                     // 1: the FP32 kernels take their ns = 8 weights from the polynomials
+};
+
+// w[k] = phi(t + k) for k < NS = 8, from v = 2 (t + NS/2) - 1 (coefficients uniform over the wave)
+template <int NS>
+__device__ __forceinline__ void es_poly_weights(float v, const EsPoly& p, float (&w)[NS]) {
+  static_assert(NS == kEsPolyNs, "the polynomial ES kernel is fitted for ns = 8");
+  const float u = v * v;
+#pragma unroll
+  for (int k = 0; k < NS / 2; ++k) {
+    float ev = p.e[k][4], od = p.o[k][4];
+#pragma unroll
+    for (int d = 3; d >= 0; --d) {
+      ev = fmaf(ev, u, p.e[k][d]);
+      od = fmaf(od, u, p.o[k][d]);
+    }
+    w[k] = fmaf(v, od, ev);
+    w[NS - 1 - k] = fmaf(-v, od, ev);
+  }
+}
+
+// v of a compact record word (l << 25 | round((t + ns/2 + 1) 2^24)): 2 (t + ns/2) - 1
+__device__ __forceinline__ float es_poly_v(unsigned word) {
+  return (float)((double)(word & 0x1FFFFFFu) * (1.0 / 8388608.0) - 3.0);
+}
+
 __device__ __forceinline__ double es_phi(double z, double beta, double c) {
   double cz2 = c * z * z;
   double a = 1.0 - cz2;

@@ -188,6 +188,11 @@ hipError_t pointset_subsort(PointSet& ps, const Geometry& g, hipStream_t s);
 void pointset_free(PointSet& ps);
 
 // ---- spreadinterp.hip
+// The polynomial ES kernel for (ns, beta, c) (common.hpp), fitted once per parameter set and
+// kept for the process; on = 0 where ns is not kEsPolyNs or the fit misses kEsPolyTol.
+// NUFFTX_ES_POLY=0 (read per call) keeps the exp form everywhere.
+EsPoly es_poly_table(int ns, double beta, double c);
+
 struct GridOpArgs {
   const PointSet* ps;
   const Geometry* g;

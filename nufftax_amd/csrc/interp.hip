@@ -31,6 +31,12 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
           auto kw = k_gather_wide<CT, D, NS, false>;
           if constexpr (D == 2)
             if (a.crec) kw = k_gather_wide<CT, D, NS, true>;
+          if constexpr (NS == kEsPolyNs)
+            if (a.poly.on) {
+              kw = k_gather_wide<CT, D, NS, false, true>;
+              if constexpr (D == 2)
+                if (a.crec) kw = k_gather_wide<CT, D, NS, true, true>;
+            }
           if (lw > 65536) (void)hipFuncSetAttribute((const void*)kw, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lw);
           hipLaunchKernelGGL(kw, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lw, s, a,
                              static_cast<const cplx<float>*>(fw), static_cast<cplx<float>*>(c));

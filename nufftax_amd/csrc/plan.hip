@@ -56,6 +56,81 @@ void kernel_params(double tol, double upsampfac, int& ns, double& beta, double& 
   c = 4.0 / (double)(ns * ns);
 }
 
+// The polynomial ES kernel (common.hpp): P_k interpolates phi(z0_k + v/2), z0_k = 1/2 + k - ns/2,
+// at the 10 Chebyshev points of [-1, 1] (FP64, degree 9), converted to monomials of v and split
+// into even and odd parts, rounded to FP32; then every interval (P_k(v) for k < ns/2,
+// P_k(-v) for the mirrored ones) is evaluated as the kernels do (FP32 FMAs) at 257 points and
+// compared with the exact phi (nufftax/core/kernel.py:79-111).
+namespace {
+EsPoly es_poly_fit(int ns, double beta, double c) {
+  EsPoly p{};
+  p.on = 0;
+  if (ns != kEsPolyNs) return p;
+  constexpr int N = 10;  // degree 9
+  auto phi = [&](double z) {
+    const double a = 1.0 - c * z * z;
+    return a >= 0.0 ? std::exp(beta * (std::sqrt(a) - 1.0)) : 0.0;
+  };
+  double tm[N][N] = {};  // tm[n][d]: coefficient of v^d in the Chebyshev polynomial T_n
+  tm[0][0] = 1.0;
+  tm[1][1] = 1.0;
+  for (int n = 2; n < N; ++n)
+    for (int d = 0; d < N; ++d) tm[n][d] = (d > 0 ? 2.0 * tm[n - 1][d - 1] : 0.0) - tm[n - 2][d];
+  for (int k = 0; k < ns / 2; ++k) {
+    const double z0 = 0.5 + k - 0.5 * ns;
+    double f[N], a[N];
+    for (int j = 0; j < N; ++j) f[j] = phi(z0 + 0.5 * std::cos(M_PI * (j + 0.5) / N));
+    for (int n = 0; n < N; ++n) {
+      double sum = 0.0;
+      for (int j = 0; j < N; ++j) sum += f[j] * std::cos(n * M_PI * (j + 0.5) / N);
+      a[n] = sum * (n == 0 ? 1.0 : 2.0) / N;
+    }
+    for (int d = 0; d < N; ++d) {
+      double m = 0.0;
+      for (int n = d; n < N; ++n) m += a[n] * tm[n][d];
+      (d % 2 == 0 ? p.e[k][d / 2] : p.o[k][d / 2]) = (float)m;
+    }
+  }
+  double maxerr = 0.0;
+  for (int q = 0; q <= 256; ++q) {
+    const float v = -1.0f + (float)q / 128.0f, u = v * v;
+    for (int k = 0; k < ns / 2; ++k) {
+      float ev = p.e[k][4], od = p.o[k][4];
+      for (int d = 3; d >= 0; --d) {
+        ev = std::fma(ev, u, p.e[k][d]);
+        od = std::fma(od, u, p.o[k][d]);
+      }
+      const double z0 = 0.5 + k - 0.5 * ns, z1 = 0.5 + (ns - 1 - k) - 0.5 * ns;
+      maxerr = std::max(maxerr, std::fabs((double)std::fma(v, od, ev) - phi(z0 + 0.5 * (double)v)));
+      maxerr = std::max(maxerr, std::fabs((double)std::fma(-v, od, ev) - phi(z1 + 0.5 * (double)v)));
+    }
+  }
+  p.on = maxerr <= kEsPolyTol ? 1 : 0;
+  return p;
+}
+}  // namespace
+
+EsPoly es_poly_table(int ns, double beta, double c) {
+  struct Entry {
+    int ns;
+    double beta, c;
+    EsPoly p;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> fits;  // one per parameter set the process has used (a few)
+  EsPoly p{};
+  p.on = 0;
+  if (ns != kEsPolyNs) return p;
+  if (const char* e = std::getenv("NUFFTX_ES_POLY"))
+    if (e[0] == '0') return p;
+  std::lock_guard<std::mutex> lk(mu);
+  for (const Entry& e : fits)
+    if (e.ns == ns && e.beta == beta && e.c == c) return e.p;
+  p = es_poly_fit(ns, beta, c);
+  if (fits.size() < 256) fits.push_back(Entry{ns, beta, c, p});
+  return p;
+}
+
 // nufftax/utils/grid.py:6-33
 int64_t next_smooth_int(int64_t n) {
   if (n <= 1) return 1;

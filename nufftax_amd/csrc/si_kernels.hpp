@@ -37,6 +37,7 @@ struct SIArgs {
   int64_t grid_cells;
   EsArgs<T> es;
   T dscale[3];  // -nf_d / (2 pi): chain factor of dz/dx
+  EsPoly poly;  // polynomial form of the FP32 kernel at ns = 8 (on = 0 elsewhere)
 };
 
 template <typename T, typename CT, int D, int NS>
@@ -777,7 +778,8 @@ __global__ void __launch_bounds__(256) k_gather_tiled(SIArgs<T> a, const cplx<T>
 // cells keep every pair 16-byte aligned), with the x weights shifted by one cell when l0 is
 // odd and zero on the extra cell; two zero cells after the tile cover the last row's overrun.
 // REC (2D): points from the compact records; otherwise folded from the coordinates.
-template <typename CT, int D, int NS, bool REC>
+// POLY: kernel weights from the polynomial ES kernel (ns = 8, common.hpp).
+template <typename CT, int D, int NS, bool REC, bool POLY = false>
 __global__ void __launch_bounds__(256) k_gather_wide(SIArgs<float> a, const cplx<float>* __restrict__ fw,
                                                      cplx<float>* __restrict__ out) {
   constexpr int NC = NS + 2;  // cells read per row
@@ -799,11 +801,15 @@ __global__ void __launch_bounds__(256) k_gather_wide(SIArgs<float> a, const cplx
   for (int j = threadIdx.x; j < sd4.z; j += blockDim.x) {
     const int p = a.perm[sd4.y + j];
     int l[3] = {0, 0, 0};
-    float t[3] = {0.0f, 0.0f, 0.0f};
+    float t[3] = {0.0f, 0.0f, 0.0f};  // t = i0 - xs, or (POLY) v = 2 (t + NS/2) - 1
     if constexpr (REC) {
       const uint2 rc = a.crec[sd4.y + j];
       unpack_compact<float, NS>(rc.x, l[0], t[0]);
       unpack_compact<float, NS>(rc.y, l[1], t[1]);
+      if constexpr (POLY) {
+        t[0] = es_poly_v(rc.x);
+        t[1] = es_poly_v(rc.y);
+      }
     } else {
       if ((unsigned)p >= (unsigned)a.M) continue;
 #pragma unroll
@@ -811,16 +817,27 @@ __global__ void __launch_bounds__(256) k_gather_wide(SIArgs<float> a, const cplx
         const Folded f = fold_coord((double)static_cast<const CT*>(a.coord[d])[p], a.nf[d]);
         const int i0 = support_start(f, NS / 2);
         l[d] = i0 - org[d];
-        t[d] = (float)((double)i0 - f.xs);
+        t[d] = POLY ? (float)(2.0 * ((double)i0 - f.xs) + (double)(NS - 1)) : (float)((double)i0 - f.xs);
       }
     }
     // t + k lies in [-NS/2, NS/2] (t in [-NS/2, 1 - NS/2)): no range select needed
     float wx[NS], wy[NS], wz[NS];
+    if constexpr (POLY) {
+      es_poly_weights<NS>(t[0], a.poly, wx);
+      es_poly_weights<NS>(t[1], a.poly, wy);
+      if constexpr (D == 3) {
+        es_poly_weights<NS>(t[2], a.poly, wz);
+      } else {
 #pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      wx[k] = es_phi_in(t[0] + (float)k, a.es.beta_log2e, a.es.c);
-      wy[k] = es_phi_in(t[1] + (float)k, a.es.beta_log2e, a.es.c);
-      wz[k] = D == 3 ? es_phi_in(t[2] + (float)k, a.es.beta_log2e, a.es.c) : 1.0f;
+        for (int k = 0; k < NS; ++k) wz[k] = 1.0f;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        wx[k] = es_phi_in(t[0] + (float)k, a.es.beta_log2e, a.es.c);
+        wy[k] = es_phi_in(t[1] + (float)k, a.es.beta_log2e, a.es.c);
+        wz[k] = D == 3 ? es_phi_in(t[2] + (float)k, a.es.beta_log2e, a.es.c) : 1.0f;
+      }
     }
     const bool odd = (l[0] & 1) != 0;
     float ws[NC];
@@ -1036,6 +1053,8 @@ SIArgs<T> make_si_args(const GridOpArgs& ga) {
   a.es.beta_log2e = std::is_same<T, float>::value ? (T)(ga.beta * log2e) : (T)ga.beta;
   a.es.beta = (T)ga.beta;
   a.es.c = (T)ga.c;
+  a.poly.on = 0;
+  if (std::is_same<T, float>::value && g.ns == kEsPolyNs) a.poly = es_poly_table(g.ns, ga.beta, ga.c);
   return a;
 }
 

@@ -405,7 +405,8 @@ __device__ __forceinline__ void split_prod_f16(const float (&a)[8], const float 
   for (int e = 0; e < 8; ++e) lo[e] = (_Float16)__builtin_fmaf(a[e], b[e], -(float)hi[e]);
 }
 
-template <int NS>
+// POLY: kernel weights from the polynomial ES kernel (ns = 8, common.hpp), else es_phi_in
+template <int NS, bool POLY>
 __global__ void __launch_bounds__(kWhThreads) k_spread_win2dh(SIArgs<float> a, int maxsub,
                                                               const uint4* __restrict__ packed,
                                                               cplx<float>* __restrict__ fw) {
@@ -576,12 +577,20 @@ __global__ void __launch_bounds__(kWhThreads) k_spread_win2dh(SIArgs<float> a, i
     // -- kernel weights of the group's points (lanes 0-31 x, 32-63 y)
     {
       const unsigned wd = sdim ? p.y : p.x;
-      const float t = (float)((double)(wd & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1));
       const float wsc = sdim ? 1.0f : 16384.0f;  // B = 2^14 wx
-      // t + m lies in [-NS/2, NS/2] for the points (t in [-NS/2, 1 - NS/2)); padding entries
-      // (t = -NS/2 - 1, zero strength) get finite weights
+      if constexpr (POLY) {
+        // padding entries (word 0, v = -3, zero strength) take v = -1: finite weights
+        float w[NS];
+        es_poly_weights<NS>(fmaxf(es_poly_v(wd), -1.0f), a.poly, w);
 #pragma unroll
-      for (int m = 0; m < NS; ++m) myrow[m] = es_phi_in(t + (float)m, bl2e, kc) * wsc;
+        for (int m = 0; m < NS; ++m) myrow[m] = w[m] * wsc;
+      } else {
+        const float t = (float)((double)(wd & 0x1FFFFFFu) * (1.0 / 16777216.0) - (double)(NS / 2 + 1));
+        // t + m lies in [-NS/2, NS/2] for the points (t in [-NS/2, 1 - NS/2)); padding entries
+        // (t = -NS/2 - 1, zero strength) get finite weights
+#pragma unroll
+        for (int m = 0; m < NS; ++m) myrow[m] = es_phi_in(t + (float)m, bl2e, kc) * wsc;
+      }
       if (sdim == 0) {
         const int lx = (int)(p.x >> 25), ly = (int)(p.y >> 25);
         const int ox = lx - min(lx / SB, 3) * SB, oy = ly - min(ly / SB, 3) * SB;  // offsets in the window

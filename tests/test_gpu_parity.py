@@ -458,15 +458,18 @@ def test_type3_small_grids_vs_direct(rng, dim, prec):
 
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
-@pytest.mark.parametrize("mfma", ["1", None, "f32win", "bank"])
+@pytest.mark.parametrize("mfma", ["1", None, "exp", "f32win", "bank"])
 def test_spread2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
     """2D FP32 spreading vs the FP64 oracle, every even ns: the default (fp16-split window kernel
-    for ns <= 8, 4096-point subproblems), the FP32 window kernel (NUFFTX_WIN_F32 with
+    for ns <= 8, 4096-point subproblems, polynomial kernel weights at ns = 8), the same with the
+    exp form of the weights (NUFFTX_ES_POLY=0), the FP32 window kernel (NUFFTX_WIN_F32 with
     1024-point subproblems), the bank-spread kernel (NUFFTX_SPREAD_WIN=0) and the records-based
     matrix-core path (NUFFTX_MFMA)."""
     from nufftax_amd._plan import F32, grid_plan
 
-    if mfma == "f32win":
+    if mfma == "exp":
+        monkeypatch.setenv("NUFFTX_ES_POLY", "0")
+    elif mfma == "f32win":
         monkeypatch.setenv("NUFFTX_WIN_F32", "1")
         monkeypatch.setenv("NUFFTX_MAXSUB", "1024")
     elif mfma == "bank":
@@ -492,14 +495,17 @@ def test_spread2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
 
 @pytest.mark.parametrize("eps", [1e-1, 1e-3, 1e-4, 1e-6])
 @pytest.mark.parametrize("dist", ["uniform", "clustered"])
-@pytest.mark.parametrize("mfma", ["1", None, "win"])
+@pytest.mark.parametrize("mfma", ["1", None, "exp", "win"])
 def test_interp2d_f32_every_nspread(rng, eps, dist, mfma, monkeypatch):
-    """2D FP32 interpolation (LDS-tiled; the records-based matrix-core kernel, NUFFTX_MFMA; the
-    fp16-split window interpolation, NUFFTX_INTERP_WIN, for ns <= 8) vs the FP64 oracle, every
-    even ns, batched."""
+    """2D FP32 interpolation (LDS-tiled, polynomial kernel weights at ns = 8; the same with the
+    exp form, NUFFTX_ES_POLY=0; the records-based matrix-core kernel, NUFFTX_MFMA; the fp16-split
+    window interpolation, NUFFTX_INTERP_WIN, for ns <= 8) vs the FP64 oracle, every even ns,
+    batched."""
     from nufftax_amd._plan import F32, grid_plan
 
-    if mfma == "win":
+    if mfma == "exp":
+        monkeypatch.setenv("NUFFTX_ES_POLY", "0")
+    elif mfma == "win":
         monkeypatch.setenv("NUFFTX_INTERP_WIN", "1")
     elif mfma:
         monkeypatch.setenv("NUFFTX_MFMA", mfma)
@@ -575,6 +581,42 @@ def test_interp3d_f32_wide_reads(rng, eps, wide, monkeypatch):
     plan.interp(_t(fw), c)
     ref = R.interp([x.astype(np.float64) for x in coords], fw.astype(np.complex128), kp)
     assert R.rel_l2(_np(c), ref) < 2e-6
+
+
+@pytest.mark.parametrize("upsampfac", [2.0, 1.25])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_es_poly_weights_vs_exp(rng, upsampfac, dim, monkeypatch):
+    """The polynomial ES kernel weights (ns = 8: degree-9 even/odd Horner forms fitted on the
+    host) against the exp form (NUFFTX_ES_POLY=0) and the FP64 oracle, at both upsampling
+    factors: spreading and interpolation within 2e-6 of the oracle, and the two forms within
+    1e-6 of each other."""
+    from nufftax_amd._plan import F32, grid_plan
+
+    kp = R.compute_kernel_params(1e-6, upsampfac)
+    assert kp.nspread == 8
+    nfs = (100, 84) if dim == 2 else (44, 40, 28)
+    M = 20000
+    coords = [np.concatenate([rng.normal(0.4, 0.1, M - 4000), rng.uniform(-np.pi, np.pi, 4000)]).astype(np.float32)
+              for _ in range(dim)]
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    shape = tuple(reversed(nfs))
+    fw = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    ref_s = R.spread([x.astype(np.float64) for x in coords], c.astype(np.complex128), nfs, kp)
+    ref_i = R.interp([x.astype(np.float64) for x in coords], fw.astype(np.complex128), kp)
+    got = {}
+    for form in ("poly", "exp"):
+        monkeypatch.setenv("NUFFTX_ES_POLY", "1" if form == "poly" else "0")
+        plan = grid_plan(dim, nfs, kp.nspread, kp.beta, kp.c, 1, F32, torch.device(DEV))
+        plan.setpts([_t(x) for x in coords])
+        g = torch.full(shape, float("nan"), dtype=torch.complex64, device=DEV)
+        plan.spread(_t(c), g)
+        v = torch.empty(M, dtype=torch.complex64, device=DEV)
+        plan.interp(_t(fw), v)
+        got[form] = (_np(g), _np(v))
+        assert R.rel_l2(got[form][0], ref_s) < 2e-6
+        assert R.rel_l2(got[form][1], ref_i) < 2e-6
+    assert R.rel_l2(got["poly"][0], got["exp"][0]) < 1e-6
+    assert R.rel_l2(got["poly"][1], got["exp"][1]) < 1e-6
 
 
 @pytest.mark.parametrize("nfs", [(77, 64), (45, 27)])
