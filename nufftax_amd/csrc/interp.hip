@@ -50,9 +50,17 @@ hipError_t interp_impl(const GridOpArgs& ga, const void* fw, void* c, hipStream_
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
                          static_cast<const cplx<T>*>(fw), o);
     } else {
-      const unsigned nblk = (unsigned)((ps.M + 255) / 256);
-      hipLaunchKernelGGL((k_gather_global<T, CT, D, NS, false>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s,
-                         a, static_cast<const cplx<T>*>(fw), o);
+      const char* gw = std::getenv("NUFFTX_GLOBAL_WAVE");
+      if (D >= 2 && !(gw && gw[0] == '0')) {
+        // one wave per point (four per block)
+        const unsigned nblk = (unsigned)((ps.M + 3) / 4);
+        hipLaunchKernelGGL((k_gather_global_wave<T, CT, D == 1 ? 2 : D, NS>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s, a,
+                           static_cast<const cplx<T>*>(fw), static_cast<cplx<T>*>(c));
+      } else {
+        const unsigned nblk = (unsigned)((ps.M + 255) / 256);
+        hipLaunchKernelGGL((k_gather_global<T, CT, D, NS, false>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s,
+                           a, static_cast<const cplx<T>*>(fw), o);
+      }
     }
     return hipGetLastError();
   });

@@ -619,6 +619,59 @@ __global__ void __launch_bounds__(256) k_spread_global(SIArgs<T> a, const cplx<T
   }
 }
 
+// Global-memory 2D/3D spread, one wave per point (grids whose tiles do not fit the LDS: the
+// sparse FP64 grids of type 3 and high-accuracy plans).  The lanes first evaluate the point's
+// D NS kernel weights at once (lane q < D NS: weight q mod NS of dimension q / NS), then every
+// atomic instruction covers R = 64 / (2 NS) support rows of one z slice, lane (r, 2 kx + part)
+// adding part (re, im) of cell kx of row r: the wave's addresses are R runs of 2 NS adjacent
+// words, a few cache lines per instruction where one point per thread touches 64 unrelated
+// ones.  Weights and support starts are those of k_spread_global (same fold, same es_phi);
+// the product is taken as c wz wx wy.
+template <typename T, typename CT, int D, int NS>
+__global__ void __launch_bounds__(256) k_spread_global_wave(SIArgs<T> a, const cplx<T>* __restrict__ c,
+                                                            cplx<T>* __restrict__ fw) {
+  constexpr int RL = 2 * NS;  // lanes per support row
+  constexpr int R = 64 / RL;  // support rows per instruction
+  static_assert((D == 2 || D == 3) && D * NS <= 64 && R >= 1, "one wave holds the D NS weights and a support row");
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // sorted slot of this wave's point
+  if (j >= a.M) return;                                                     // uniform over the wave
+  const int tr = blockIdx.y;
+  const int p = a.perm[j];
+  if ((unsigned)p >= (unsigned)a.M) return;
+  const int dq = lane / NS, kq = lane - dq * NS;
+  T wq = (T)0;
+  int i0q = 0;
+  if (dq < D) {
+    const double x = (double)static_cast<const CT*>(a.coord[dq])[p];
+    const Folded f = fold_coord(x, a.nf[dq]);
+    i0q = support_start(f, NS / 2);
+    wq = es_phi((T)((double)i0q - f.xs) + (T)kq, a.es.beta_log2e, a.es.c);
+  }
+  const int i0x = __shfl(i0q, 0, 64), i0y = __shfl(i0q, NS, 64), i0z = __shfl(i0q, D == 3 ? 2 * NS : 0, 64);
+  const int rr = lane / RL, q = lane - rr * RL, kx = q >> 1, part = q & 1;
+  const T wx = __shfl(wq, kx < NS ? kx : 0, 64);
+  const cplx<T> cv = c[(int64_t)tr * a.M + p];
+  const T cx = (part ? cv.im : cv.re) * wx;
+  const int64_t gx = wrap_idx(i0x + kx, a.nf[0]);
+  T* fT = reinterpret_cast<T*>(fw + (int64_t)tr * a.grid_cells);
+#pragma unroll 1
+  for (int kz = 0; kz < (D == 3 ? NS : 1); ++kz) {
+    const T wz = D == 3 ? __shfl(wq, 2 * NS + kz, 64) : (T)1;
+    const int64_t zoff = D == 3 ? (int64_t)wrap_idx(i0z + kz, a.nf[2]) * a.nf[1] : 0;
+    const T cz = cx * wz;
+#pragma unroll
+    for (int r0 = 0; r0 < NS; r0 += R) {
+      const int ky = r0 + rr;
+      const T wy = __shfl(wq, NS + (ky < NS ? ky : NS - 1), 64);
+      if (rr < R && ky < NS) {
+        const int64_t cell = (zoff + wrap_idx(i0y + ky, a.nf[1])) * a.nf[0] + gx;
+        atomic_add_global(&fT[2 * cell + part], cz * wy);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ gather helpers
 // Accumulates sum over the support of G[cell] * wx[kx] * wy[ky] * wz[kz] for
 // value weights, and optionally the per-dimension derivative sums.
@@ -1010,6 +1063,62 @@ __global__ void __launch_bounds__(256) k_gather_global(SIArgs<T> a, const cplx<T
   cplx<T> s0, sdd[D];
   gather_point<T, D, NS, DERIV>(w, dw, fetch, s0, sdd);
   emit_point<T, D, DERIV>(a, o, tr, p, s0, sdd);
+}
+
+// Global-memory 2D/3D interpolation, one wave per point (the transpose of
+// k_spread_global_wave): after the weights, every load instruction covers R = 64 / (2 NS)
+// support rows of one z slice, lane (r, 2 kx + part) reading part (re, im) of cell kx of row r
+// (R runs of 2 NS adjacent words), and multiplies it by its wx wy wz; the real and imaginary
+// sums end on lanes 0 and 1 after a butterfly over the lanes of equal parity.
+template <typename T, typename CT, int D, int NS>
+__global__ void __launch_bounds__(256) k_gather_global_wave(SIArgs<T> a, const cplx<T>* __restrict__ fw,
+                                                            cplx<T>* __restrict__ out) {
+  constexpr int RL = 2 * NS;  // lanes per support row
+  constexpr int R = 64 / RL;  // support rows per instruction
+  static_assert((D == 2 || D == 3) && D * NS <= 64 && R >= 1, "one wave holds the D NS weights and a support row");
+  const int lane = (int)(threadIdx.x & 63);
+  const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // sorted slot of this wave's point
+  if (j >= a.M) return;                                                     // uniform over the wave
+  const int tr = blockIdx.y;
+  const int p = a.perm[j];
+  if ((unsigned)p >= (unsigned)a.M) return;
+  const int dq = lane / NS, kq = lane - dq * NS;
+  T wq = (T)0;
+  int i0q = 0;
+  if (dq < D) {
+    const double x = (double)static_cast<const CT*>(a.coord[dq])[p];
+    const Folded f = fold_coord(x, a.nf[dq]);
+    i0q = support_start(f, NS / 2);
+    wq = es_phi((T)((double)i0q - f.xs) + (T)kq, a.es.beta_log2e, a.es.c);
+  }
+  const int i0x = __shfl(i0q, 0, 64), i0y = __shfl(i0q, NS, 64), i0z = __shfl(i0q, D == 3 ? 2 * NS : 0, 64);
+  const int rr = lane / RL, q = lane - rr * RL, kx = q >> 1, part = q & 1;
+  const T wx = __shfl(wq, kx < NS ? kx : 0, 64);
+  const int64_t gx = wrap_idx(i0x + kx, a.nf[0]);
+  const T* fT = reinterpret_cast<const T*>(fw + (int64_t)tr * a.grid_cells);
+  T acc = (T)0;
+#pragma unroll 1
+  for (int kz = 0; kz < (D == 3 ? NS : 1); ++kz) {
+    const T wz = D == 3 ? __shfl(wq, 2 * NS + kz, 64) : (T)1;
+    const int64_t zoff = D == 3 ? (int64_t)wrap_idx(i0z + kz, a.nf[2]) * a.nf[1] : 0;
+    T accz = (T)0;
+#pragma unroll
+    for (int r0 = 0; r0 < NS; r0 += R) {
+      const int ky = r0 + rr;
+      const T wy = __shfl(wq, NS + (ky < NS ? ky : NS - 1), 64);
+      if (rr < R && ky < NS) {
+        const int64_t cell = (zoff + wrap_idx(i0y + ky, a.nf[1])) * a.nf[0] + gx;
+        accz += fT[2 * cell + part] * wy;
+      }
+    }
+    acc += accz * wz;
+  }
+  acc *= wx;
+  // lanes of equal parity (2 kx + part keeps part = lane & 1, RL being even) hold one part
+#pragma unroll
+  for (int off = 2; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+  const T im = __shfl(acc, 1, 64);
+  if (lane == 0) out[(int64_t)tr * a.M + p] = cplx<T>{acc, im};
 }
 
 // ------------------------------------------------------------------ host-side helpers

@@ -75,9 +75,17 @@ hipError_t spread_impl(const GridOpArgs& ga, const void* c, void* fw, hipStream_
       hipLaunchKernelGGL(kern, dim3(ps.max_subprob, (unsigned)ga.n_trans), dim3(256), lds, s, a,
                          static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
     } else {
-      const unsigned nblk = (unsigned)((ps.M + 255) / 256);
-      hipLaunchKernelGGL((k_spread_global<T, CT, D, NS>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s, a,
-                         static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
+      const char* gw = std::getenv("NUFFTX_GLOBAL_WAVE");
+      if (D >= 2 && !(gw && gw[0] == '0')) {
+        // one wave per point (four per block)
+        const unsigned nblk = (unsigned)((ps.M + 3) / 4);
+        hipLaunchKernelGGL((k_spread_global_wave<T, CT, D == 1 ? 2 : D, NS>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s, a,
+                           static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
+      } else {
+        const unsigned nblk = (unsigned)((ps.M + 255) / 256);
+        hipLaunchKernelGGL((k_spread_global<T, CT, D, NS>), dim3(nblk, (unsigned)ga.n_trans), dim3(256), 0, s, a,
+                           static_cast<const cplx<T>*>(c), static_cast<cplx<T>*>(fw));
+      }
     }
     return hipGetLastError();
   });

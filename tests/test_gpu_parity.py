@@ -166,6 +166,45 @@ def test_spread_global_path_matches_tiled(rng, monkeypatch):
     assert R.rel_l2(_np(fw), ref) < 1e-12
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("eps", [1e-3, 1e-6, 1e-10])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_global_path_wave_vs_thread(rng, dim, prec, eps, monkeypatch):
+    """2D and 3D spreading and interpolation on the global-memory path (tiles above
+    NUFFTX_LDS_TILE_MAX): one wave per point (default) and one thread per point
+    (NUFFTX_GLOBAL_WAVE=0), each against the FP64 oracle, and against each other; two
+    transforms, clustered and spread-out points, grids with wrapped supports."""
+    from nufftax_amd._plan import F32, F64, grid_plan
+
+    if prec == "f32" and eps < 1e-7:
+        pytest.skip("FP32 plans take eps >= 1e-7")
+    monkeypatch.setenv("NUFFTX_LDS_TILE_MAX", "4096")
+    kp = R.compute_kernel_params(eps)
+    nfs = (36, 30, 26)[:dim] if dim == 3 else (120, 90)
+    M = 4000
+    coords = [np.concatenate([rng.normal(3.0, 0.1, M - 1500), rng.uniform(-np.pi, np.pi, 1500)]) for _ in range(dim)]
+    c = rng.standard_normal((2, M)) + 1j * rng.standard_normal((2, M))
+    g = rng.standard_normal((2,) + tuple(reversed(nfs))) + 1j * rng.standard_normal((2,) + tuple(reversed(nfs)))
+    cdt, rdt, pr = (np.complex64, np.float32, F32) if prec == "f32" else (np.complex128, np.float64, F64)
+    tol = 2e-6 if prec == "f32" else 1e-12
+    ref_s = R.spread(coords, c, nfs, kp)
+    ref_i = R.interp(coords, g, kp)
+    got = {}
+    for wave in ("1", "0"):
+        monkeypatch.setenv("NUFFTX_GLOBAL_WAVE", wave)
+        plan = grid_plan(dim, nfs, kp.nspread, kp.beta, kp.c, 2, pr, torch.device(DEV))
+        plan.setpts([_t(x.astype(rdt)) for x in coords])
+        fw = torch.full((2,) + tuple(reversed(nfs)), float("nan"), dtype=_t(c.astype(cdt)).dtype, device=DEV)
+        plan.spread(_t(c.astype(cdt)), fw)
+        out = torch.empty((2, M), dtype=fw.dtype, device=DEV)
+        plan.interp(_t(g.astype(cdt)), out)
+        got[wave] = (_np(fw), _np(out))
+        assert R.rel_l2(got[wave][0], ref_s) < tol
+        assert R.rel_l2(got[wave][1], ref_i) < tol
+    assert R.rel_l2(got["1"][0], got["0"][0]) < tol
+    assert R.rel_l2(got["1"][1], got["0"][1]) < tol
+
+
 # ------------------------------------------------------------------ transforms vs direct NUDFT
 
 
