@@ -11,9 +11,9 @@ batching of ``nufftax/core/spread.py:100-173`` and the batcher fast path of
   fit while later ones do (those go ahead of it, a bounded number of times), and batches
   what is queued:
   requests with the same plan signature that share their point set become one
-  n_trans unit; with ``max_units`` > 1 and at least ``batch_min_queue`` requests
+  n_trans unit; with ``max_units`` > 1 (default 8) and at least ``batch_min_queue`` requests
   queued, up to ``max_units`` units of one kind, dimension, tolerance, sign and precision with
-  distinct grid sizes (any points, M up to ``batch_max_points``) run through
+  distinct grid sizes (any points, M up to ``batch_max_points``, eps below 1e-4) run through
   ``nufftx_execute_many``, which gives them one shared spreading (type 1) or
   interpolation (type 2) launch; each batch is issued on one of ``n_streams`` HIP
   streams (round robin),
@@ -100,7 +100,7 @@ class NufftServer:
     """Asynchronous NUFFT executor over ``n_streams`` HIP streams with HBM admission control."""
 
     def __init__(self, device="cuda:0", n_streams: int = 4, hbm_budget_bytes: int = 32 << 30, max_group: int = 64,
-                 max_units: int = 1, batch_max_points: int = 200_000):
+                 max_units: int = 8, batch_max_points: int = 200_000):
         if not torch.cuda.is_available():
             raise _lib.NufftxError("NufftServer needs an AMD GPU")
         self.device = torch.device(device)
@@ -258,7 +258,9 @@ class NufftServer:
         return units, held
 
     def _batchable(self, r: ServeRequest) -> bool:
-        return r.coords[0].shape[-1] <= self.batch_max_points
+        # small point sets only, and tolerances below 1e-4: there a plan called alone also
+        # spreads into 64-bit tiles (above it, into packed ones, which shared launches do not use)
+        return r.coords[0].shape[-1] <= self.batch_max_points and r.eps < 1e-4
 
     def _dispatch_loop(self):
         pending: list = []  # queued requests in arrival order
@@ -442,7 +444,7 @@ def _direct_call(kind, coords, data, nm, eps=1e-6):
     return fn(*coords, data, eps=eps)
 
 
-def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4, max_units: int = 1):
+def run_stream(reqs, qps: float, seed: int = 0, mode: str = "server", n_streams: int = 4, max_units: int = 8):
     """Replay reqs with Poisson arrivals at qps (qps <= 0: every request arrives at once, which
     measures the saturated throughput).  mode "server": NufftServer (max_units point sets per
     shared launch; 1 turns cross-point batching off); mode "per-call": the same transforms called
@@ -568,7 +570,7 @@ def main(argv=None):
                     help="one stream replay per rate; 0 = all requests at once (saturated throughput)")
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--max-units", type=int, default=1,
+    ap.add_argument("--max-units", type=int, default=8,
                     help="point sets per shared spreading / interpolation launch in the server (1..8)")
     ap.add_argument("--compare-batched", type=int, default=0, metavar="UNITS",
                     help="also replay through a server with this many point sets per shared launch")
