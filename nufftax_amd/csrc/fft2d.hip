@@ -14,13 +14,18 @@
 // holds only nm0 of the nf0 x-bins, transposed so that both passes read and write whole
 // sequences; a block handles S sequences (S 8-byte runs on the transposed side).
 //
-// Each sequence of N = R1 R2 points is transformed in LDS by the four-step algorithm: R1
-// register DFTs of R2 points over the stride-R1 subsequences, the twiddles W_N^(n1 k2), then
-// R2 register DFTs of R1 points over contiguous runs, written back in natural order
-// (X[k2 + R2 k1]).  A register DFT is a radix-2 decimation-in-time FFT on bit-reversed
-// loads.  Twiddles exp(-2 pi i k / N), k < N/2, come from a table computed on the host in
-// FP64 (staged in LDS) and are conjugated for isign > 0.  A block of kFftThreads threads
-// transforms S = kFftThreads / R1 sequences (R1 >= R2).
+// Each sequence of N points is transformed in LDS by the Stockham autosort FFT (Govindaraju
+// et al., "High performance discrete Fourier transforms on graphics processors", SC 2008) in
+// 2 or 3 passes of radix 8 or 16 (1024 = 16 x 8 x 8): pass p reads x[j + r N/R] (r < R), scales
+// them by W_(Ns R)^(r (j mod Ns)), takes an R-point register DFT and writes it to
+// (j / Ns) Ns R + (j mod Ns) + r Ns, Ns being the product of the earlier radices; the output of
+// the last pass is in natural order.  A register DFT is a radix-2 decimation-in-time FFT on
+// bit-reversed loads.  Twiddles exp(-2 pi i k / N), k < N/2, come from a table computed on the
+// host in FP64 (staged in LDS) and are conjugated for isign > 0.  A block of kFftThreads
+// threads transforms S sequences with T = min(N / 8, 128) threads each (one 8-point column of
+// a radix-8 pass per thread).  An earlier four-step version (two passes of 32-point register
+// DFTs, 32 threads per 1024-point sequence) left most of the GPU idle: 128 blocks of two waves
+// for the 512 kept columns of a 1024^2 grid.
 #include <cmath>
 #include <vector>
 
@@ -30,7 +35,13 @@ namespace nufftx {
 
 namespace {
 
-constexpr int kFftThreads = 128;
+constexpr int kFftThreads = 256;
+
+// threads per sequence of 2^logn points, and sequences per block
+__host__ __device__ constexpr int fft_seq_threads(int logn) {
+  return (1 << logn) / 8 < 8 ? 8 : ((1 << logn) / 8 > 128 ? 128 : (1 << logn) / 8);
+}
+__host__ __device__ constexpr int fft_seqs(int logn) { return kFftThreads / fft_seq_threads(logn); }
 
 __device__ __forceinline__ int fft_mode_to_bin(int m, int n, int nf, int modeord) {
   const int kmin = -(n / 2), kmax = (n - 1) / 2;
@@ -57,8 +68,8 @@ __host__ __device__ constexpr int bitrev_c(int i) {
   return r;
 }
 
-// LDS layout of a sequence of N = 2^LOGN points: one pad slot after every R1 points, so that
-// the contiguous runs read in step B of fft_4step start on different banks
+// LDS layout of a sequence of N = 2^LOGN points: one pad slot after every 2^ceil(LOGN / 2)
+// points, which spreads the strided writes of the first Stockham passes over the banks
 template <int LOGN>
 struct FftLayout {
   static constexpr int L1 = (LOGN + 1) / 2;
@@ -89,51 +100,55 @@ __device__ __forceinline__ void reg_dft(float2 (&v)[1 << L], const float2* tw, i
   }
 }
 
-// In-place FFT of S = kFftThreads / R1 sequences of N = 2^(L1 + L2) points in the padded
-// layout FftLayout (natural order in and out).  Ends with a barrier.
-template <int L1, int L2>
-__device__ __forceinline__ void fft_4step(float2* buf, const float2* tw, bool conj) {
-  constexpr int R1 = 1 << L1, R2 = 1 << L2, N = R1 * R2;
-  constexpr int S = kFftThreads / R1;
-  using Lay = FftLayout<L1 + L2>;
-  // step A: thread (r, n1) transforms x[n1 + R1 n2] over n2 and scales by W_N^(n1 k2)
-  {
-    const int r = threadIdx.x / R1, n1 = threadIdx.x % R1;
-    float2 v[R2];
+// One Stockham pass of radix R = 2^L over the S sequences of a block (in place in the padded
+// layout: every read of the pass precedes every write); Ns = 2^LNS.  Ends with a barrier.
+template <int LOGN, int L, int LNS>
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, bool conj) {
+  constexpr int N = 1 << LOGN, R = 1 << L, NR = N / R, Ns = 1 << LNS;
+  constexpr int T = fft_seq_threads(LOGN);
+  constexpr int J = (NR + T - 1) / T;  // columns per thread
+  using Lay = FftLayout<LOGN>;
+  const int seq = threadIdx.x / T, q = threadIdx.x % T;
+  float2 v[J][R];
 #pragma unroll
-    for (int n2 = 0; n2 < R2; ++n2) v[bitrev_c<L2>(n2)] = buf[Lay::at(r, n1 + R1 * n2)];
-    reg_dft<L2>(v, tw, N, conj);
+  for (int jj = 0; jj < J; ++jj) {
+    const int j = q + jj * T;
+    if (j >= NR) break;
+    const int k = j & (Ns - 1);
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2)
-      buf[Lay::at(r, n1 + R1 * k2)] = k2 == 0 ? v[0] : cmul(v[k2], twiddle(tw, n1 * k2, N, conj));
+    for (int r = 0; r < R; ++r) {
+      const float2 x = buf[Lay::at(seq, j + r * NR)];
+      v[jj][bitrev_c<L>(r)] = (r == 0 || LNS == 0) ? x : cmul(x, twiddle(tw, r * k * (N / (Ns * R)), N, conj));
+    }
+    reg_dft<L>(v[jj], tw, N, conj);
   }
   __syncthreads();
-  // step B: thread (r, k2) transforms y[n1 + R1 k2] over n1 into X[k2 + R2 k1]; every read
-  // precedes every write (the runs read and the slots written belong to different threads)
-  float2 v[R1];
-  const bool act = (int)threadIdx.x < S * R2;
-  const int r = threadIdx.x / R2, k2 = threadIdx.x % R2;
-  if (act) {
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) v[bitrev_c<L1>(n1)] = buf[Lay::at(r, n1 + R1 * k2)];
-    reg_dft<L1>(v, tw, N, conj);
-  }
-  __syncthreads();
-  if (act) {
+  for (int jj = 0; jj < J; ++jj) {
+    const int j = q + jj * T;
+    if (j >= NR) break;
+    const int k = j & (Ns - 1);
+    const int d = (j >> LNS) * Ns * R + k;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) buf[Lay::at(r, k2 + R2 * k1)] = v[k1];
+    for (int r = 0; r < R; ++r) buf[Lay::at(seq, d + r * Ns)] = v[jj][r];
   }
   __syncthreads();
 }
 
-// N = 2^LOGN, 6 <= LOGN <= 12: R1 = 2^ceil(LOGN / 2), R2 = 2^floor(LOGN / 2)
+// In-place FFT of the S sequences of N = 2^LOGN points (6 <= LOGN <= 12) of a block, natural
+// order in and out: ceil(LOGN / 4) passes of 3 or 4 bits (the first ones the larger).  Ends
+// with a barrier.
 template <int LOGN>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, bool conj) {
-  fft_4step<(LOGN + 1) / 2, LOGN / 2>(buf, tw, conj);
+  constexpr int NP = (LOGN + 3) / 4;
+  constexpr int B0 = LOGN / NP + (0 < LOGN % NP ? 1 : 0);
+  constexpr int B1 = LOGN / NP + (1 < LOGN % NP ? 1 : 0);
+  constexpr int B2 = LOGN - B0 - B1;
+  static_assert(NP >= 2 && NP <= 3 && B0 <= 4 && B1 <= 4 && B2 <= 4, "2 or 3 passes of radix <= 16");
+  stockham_pass<LOGN, B0, 0>(buf, tw, conj);
+  stockham_pass<LOGN, B1, B0>(buf, tw, conj);
+  if constexpr (B2 > 0) stockham_pass<LOGN, B2, B0 + B1>(buf, tw, conj);
 }
-
-// sequences per block: kFftThreads / R1
-__host__ __device__ constexpr int fft_seqs(int logn) { return kFftThreads >> ((logn + 1) / 2); }
 
 __device__ __forceinline__ void load_twiddles(float2* stw, const float2* __restrict__ tw, int n) {
   for (int k = threadIdx.x; k < (n >> 1); k += kFftThreads) stw[k] = tw[k];

@@ -675,17 +675,17 @@ int nufftx_makeplan(int type, int dim, const int64_t* n_modes, int isign, int64_
       if ((r = upload_factors(p->fac[d], deconv_factors(ph, p->n_modes[d], modeord), p->prec))) goto fail_out;
     }
     if ((r = fft_make(p->fft, dim, p->g.nf, n_trans, p->prec))) goto fail_out;
-    // Fused FFT + deconvolution passes on 1024^2 (MI355X): modes -> grid 27 us against 33 us
-    // for rocFFT + the padding kernel, grid -> modes 39 us against 31 us for rocFFT + the
-    // shuffle kernel.  So the pad direction is fused by default; NUFFTX_FUSED_FFT fuses both
-    // directions, NUFFTX_ROCFFT neither.
+    // Fused FFT + deconvolution passes (Stockham, fft2d.hip) in both directions: on the 2D
+    // radial 4M workload (1024^2 grid) nufft2d1 0.198 -> 0.187 ms against rocFFT + the shuffle
+    // kernel, nufft2d2 0.160 -> 0.156 ms against the earlier four-step passes.  NUFFTX_ROCFFT
+    // takes rocFFT + the separate deconvolution kernels in both directions.
     if (fft2_applicable(dargs(p, false)) && !std::getenv("NUFFTX_ROCFFT")) {
       if (fft2_plan_init(p->fft2, dargs(p, false)) != hipSuccess) {
         r = fail(NUFFTX_ERR_ALLOC, "fused FFT twiddle tables and intermediate buffer");
         goto fail_out;
       }
       p->fft2_pad = true;
-      p->fft2_shuffle = std::getenv("NUFFTX_FUSED_FFT") != nullptr;
+      p->fft2_shuffle = true;
     }
   } else {
     char* gi = nullptr;

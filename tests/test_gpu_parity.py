@@ -685,14 +685,14 @@ def test_interp2d_f32_wide_reads(rng, nfs, eps, wide, monkeypatch):
 # ------------------------------------------------------------------ fused FFT + deconvolution
 
 
-@pytest.mark.parametrize("nm", [(63, 31), (256, 128), (512, 512)])
+@pytest.mark.parametrize("nm", [(32, 32), (63, 31), (256, 128), (512, 512), (2048, 1024)])
 @pytest.mark.parametrize("isign", [1, -1])
 @pytest.mark.parametrize("modeord", [0, 1])
 def test_fused_fft_matches_rocfft(rng, monkeypatch, nm, isign, modeord):
-    """Plans built with NUFFTX_FUSED_FFT run FFT + deconvolution of 2D FP32 power-of-two grids
-    as two fused passes in both directions; the same plans built with NUFFTX_ROCFFT take
-    rocFFT + the separate deconvolution kernels.  Type 1 and type 2, two transforms per call;
-    relative l2 < 2e-6 (FP32 FFT rounding)."""
+    """Plans of 2D FP32 power-of-two grids (64 .. 4096 cells a side here) run FFT +
+    deconvolution as two fused Stockham passes in both directions (default); the same plans
+    built with NUFFTX_ROCFFT take rocFFT + the separate deconvolution kernels.  Type 1 and
+    type 2, two transforms per call; relative l2 < 2e-6 (FP32 FFT rounding)."""
     from nufftax_amd._plan import F32, Plan
 
     M = 20000
@@ -703,11 +703,9 @@ def test_fused_fft_matches_rocfft(rng, monkeypatch, nm, isign, modeord):
 
     def run(rocfft):
         if rocfft:
-            monkeypatch.delenv("NUFFTX_FUSED_FFT", raising=False)
             monkeypatch.setenv("NUFFTX_ROCFFT", "1")
         else:
             monkeypatch.delenv("NUFFTX_ROCFFT", raising=False)
-            monkeypatch.setenv("NUFFTX_FUSED_FFT", "1")
         p1 = Plan.transform(1, 2, nm, 2, 1e-6, isign, 2.0, F32, modeord, torch.device(DEV))
         p2 = Plan.transform(2, 2, nm, 2, 1e-6, -isign, 2.0, F32, modeord, torch.device(DEV))
         assert p1.grid_size(0) == 2 ** int(np.log2(p1.grid_size(0)))
