@@ -141,27 +141,33 @@ __device__ __forceinline__ void seg_prefix_bin(const ScanArgs& sa, int b) {
 }
 
 // one block: bin totals (in bin_start) -> exclusive scan = bin_start; subproblems per bin ->
-// sub_start; bin_start[nbins] = M; nsub
+// sub_start; bin_start[nbins] = M; nsub.  Each thread takes a run of consecutive bins (16 at
+// 16384 bins and 1024 threads): its sums, one block scan, then its run written out.
 __device__ __forceinline__ void bin_scan(const ScanArgs& sa) {
-  int carry = 0, carry_sub = 0;
-  for (int base = 0; base < sa.nbins; base += blockDim.x) {
-    const int b = base + threadIdx.x;
-    const int cnt = b < sa.nbins ? sa.bin_start[b] : 0;
-    const int nsb = (cnt + sa.maxsub - 1) / sa.maxsub;
-    int total, total_sub;
-    const int ex = block_excl_scan(cnt, total) + carry;
-    const int ex_sub = block_excl_scan(nsb, total_sub) + carry_sub;
-    if (b < sa.nbins) {
-      sa.bin_start[b] = ex;
-      sa.sub_start[b] = ex_sub;
-    }
-    carry += total;
-    carry_sub += total_sub;
+  const int per = (sa.nbins + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int b0 = (int)threadIdx.x * per, b1 = min(b0 + per, sa.nbins);
+  int cnt = 0, nsb = 0;
+#pragma unroll 4
+  for (int b = b0; b < b1; ++b) {
+    const int c = sa.bin_start[b];
+    cnt += c;
+    nsb += (c + sa.maxsub - 1) / sa.maxsub;
+  }
+  int total, total_sub;
+  int ex = block_excl_scan(cnt, total);
+  int ex_sub = block_excl_scan(nsb, total_sub);
+#pragma unroll 4
+  for (int b = b0; b < b1; ++b) {
+    const int c = sa.bin_start[b];  // this thread's own entries: read before it overwrites them
+    sa.bin_start[b] = ex;
+    sa.sub_start[b] = ex_sub;
+    ex += c;
+    ex_sub += (c + sa.maxsub - 1) / sa.maxsub;
   }
   if (threadIdx.x == 0) {
     sa.bin_start[sa.nbins] = sa.M;
-    sa.sub_start[sa.nbins] = carry_sub;
-    *sa.nsub = min(carry_sub, sa.max_subprob);
+    sa.sub_start[sa.nbins] = total_sub;
+    *sa.nsub = min(total_sub, sa.max_subprob);
   }
 }
 
