@@ -1,5 +1,5 @@
-// Fused uniform-grid stage of 2D FP32 type-1 and type-2 transforms on power-of-two fine
-// grids (64 .. 4096 per axis): the c2c FFT of the fine grid (nufftax/transforms/nufft1.py:81,
+// Fused uniform-grid stage of 2D and 3D FP32 type-1 and type-2 transforms on power-of-two fine
+// grids (64 .. 4096 per axis; 3D: k_fft3_pass, three passes, described there): the c2c FFT of the fine grid (nufftax/transforms/nufft1.py:81,
 // nufft2.py:74: fftn for isign < 0, unnormalised ifftn * N for isign > 0) together with the
 // deconvolution, mode truncation and shift of deconvolve.py:174-224 (type 1) or the
 // deconvolution and zero padding of deconvolve.py:227-304 (type 2).
@@ -26,7 +26,9 @@
 // a radix-8 pass per thread).  An earlier four-step version (two passes of 32-point register
 // DFTs, 32 threads per 1024-point sequence) left most of the GPU idle: 128 blocks of two waves
 // for the 512 kept columns of a 1024^2 grid.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "internal.hpp"
@@ -301,6 +303,77 @@ __global__ void __launch_bounds__(kFftThreads) k_fft2_rows_t2(Fft2K k, const flo
   }
 }
 
+// One pass of the 3D fused FFT: S sequences per block along one axis, read contiguously from
+// `in` and written with stride `ostride` to `out` (consecutive sequences to consecutive
+// addresses, so a block writes runs of S values).  Sequence seq = a D1 + b starts at
+// a P1 + b P2 in `in` and a Q1 + b Q2 in `out`.  Inputs are either N grid values or the nin
+// modes of this axis (placed at their FFT bins, the rest zero); outputs are the N bins or the
+// nout kept modes of this axis.  fac: facA[mode of this axis] facB[b] facC[a] (each optional)
+// on the modes side.
+struct Fft3Pass {
+  int nin, nout, nmode, modeord;
+  int in_modes, out_modes;
+  int64_t nseq, D1, P1, P2, Q1, Q2, ostride, in_t, out_t;
+  const float* facA;
+  const float* facB;
+  const float* facC;
+  const float2* tw;
+};
+
+template <int LOGN>
+__global__ void __launch_bounds__(kFftThreads) k_fft3_pass(Fft3Pass P, const float2* __restrict__ in,
+                                                           float2* __restrict__ out, int conj) {
+  extern __shared__ __align__(16) float2 smem[];
+  constexpr int N = 1 << LOGN, S = fft_seqs(LOGN);
+  using Lay = FftLayout<LOGN>;
+  __shared__ int64_t sa[S], sb[S];
+  float2* stw = smem;
+  float2* buf = smem + (N >> 1);
+  const int64_t s0 = (int64_t)blockIdx.x * S;
+  const int64_t t = blockIdx.y;
+  const float2* inT = in + t * P.in_t;
+  float2* outT = out + t * P.out_t;
+  if ((int)threadIdx.x < S) {
+    const int64_t seq = s0 + threadIdx.x;
+    sa[threadIdx.x] = seq / P.D1;
+    sb[threadIdx.x] = seq % P.D1;
+  }
+  load_twiddles(stw, P.tw, N);
+  if (P.in_modes)
+    for (int e = threadIdx.x; e < S * Lay::PL; e += kFftThreads) buf[e] = make_float2(0.f, 0.f);
+  __syncthreads();
+  for (int e = threadIdx.x; e < S * P.nin; e += kFftThreads) {
+    const int r = e / P.nin, i = e - r * P.nin;
+    if (s0 + r >= P.nseq) continue;
+    float2 v = inT[sa[r] * P.P1 + sb[r] * P.P2 + i];
+    if (P.in_modes) {
+      float f = P.facA ? P.facA[i] : 1.0f;
+      if (P.facB) f *= P.facB[sb[r]];
+      if (P.facC) f *= P.facC[sa[r]];
+      buf[Lay::at(r, fft_mode_to_bin(i, P.nmode, N, P.modeord))] = make_float2(v.x * f, v.y * f);
+    } else {
+      buf[Lay::at(r, i)] = v;
+    }
+  }
+  __syncthreads();
+  lds_fft<LOGN>(buf, stw, conj != 0);
+  for (int e = threadIdx.x; e < S * P.nout; e += kFftThreads) {
+    const int j = e / S, r = e - j * S;
+    if (s0 + r >= P.nseq) continue;
+    float2 v;
+    if (P.out_modes) {
+      v = buf[Lay::at(r, fft_mode_to_bin(j, P.nmode, N, P.modeord))];
+      float f = P.facA ? P.facA[j] : 1.0f;
+      if (P.facB) f *= P.facB[sb[r]];
+      if (P.facC) f *= P.facC[sa[r]];
+      v = make_float2(v.x * f, v.y * f);
+    } else {
+      v = buf[Lay::at(r, j)];
+    }
+    outT[sa[r] * P.Q1 + sb[r] * P.Q2 + (int64_t)j * P.ostride] = v;
+  }
+}
+
 int log2_exact(int n) {
   int l = 0;
   while ((1 << l) < n) ++l;
@@ -342,18 +415,37 @@ void fft_lds_attr(const void* kern, size_t b) {
 }  // namespace
 
 bool fft2_applicable(const DeconvArgs& a) {
-  if (a.dim != 2 || a.prec != kF32 || a.from_phihat) return false;
-  for (int d = 0; d < 2; ++d) {
+  if ((a.dim != 2 && a.dim != 3) || a.prec != kF32 || a.from_phihat) return false;
+  if (a.dim == 3)
+    if (const char* e = std::getenv("NUFFTX_FFT3_FUSED"))
+      if (e[0] == '0') return false;
+  for (int d = 0; d < a.dim; ++d) {
     const int l = log2_exact(a.nf[d]);
     if (l < 6 || l > 12 || a.nm[d] > a.nf[d]) return false;
   }
   return true;
 }
 
+// intermediate sizes of the 3D passes per transform: type 2 writes [n2][nf0][n1] then
+// [nf1][nf0][n2], type 1 [nf2][n0][nf1] then [n1][n0][nf2]
+int64_t fft3_mid_cells(const DeconvArgs& a) {
+  const int64_t n0 = a.nm[0], n1 = a.nm[1], n2 = a.nm[2], f0 = a.nf[0], f1 = a.nf[1], f2 = a.nf[2];
+  return std::max(n2 * f0 * n1, f2 * n0 * f1);
+}
+int64_t fft3_mid2_cells(const DeconvArgs& a) {
+  const int64_t n0 = a.nm[0], n1 = a.nm[1], n2 = a.nm[2], f0 = a.nf[0], f1 = a.nf[1], f2 = a.nf[2];
+  return std::max(f1 * f0 * n2, n1 * n0 * f2);
+}
+
 hipError_t fft2_plan_init(Fft2Plan& fp, const DeconvArgs& a) {
-  hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * (int64_t)a.nm[0] * a.nf[1], fp.retired);
-  if (e != hipSuccess) return e;
-  for (int d = 0; d < 2; ++d) {
+  hipError_t e;
+  if (a.dim == 3) {
+    if ((e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * fft3_mid_cells(a), fp.retired)) != hipSuccess) return e;
+    if ((e = grow_buffer(fp.mid2, fp.cap_mid2, a.n_trans * fft3_mid2_cells(a), fp.retired)) != hipSuccess) return e;
+  } else if ((e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * (int64_t)a.nm[0] * a.nf[1], fp.retired)) != hipSuccess) {
+    return e;
+  }
+  for (int d = 0; d < a.dim; ++d) {
     const int n = a.nf[d];
     std::vector<float2> h((size_t)n / 2);
     for (int k = 0; k < n / 2; ++k) {
@@ -372,15 +464,17 @@ hipError_t fft2_plan_init(Fft2Plan& fp, const DeconvArgs& a) {
 }
 
 void fft2_plan_free(Fft2Plan& fp) {
-  for (int d = 0; d < 2; ++d) {
+  for (int d = 0; d < 3; ++d) {
     if (fp.tw[d]) (void)hipFree(fp.tw[d]);
     fp.tw[d] = nullptr;
   }
   for (void* p : fp.retired) (void)hipFree(p);
   fp.retired.clear();
   if (fp.mid) (void)hipFree(fp.mid);
+  if (fp.mid2) (void)hipFree(fp.mid2);
   fp.mid = nullptr;
-  fp.cap_mid = 0;
+  fp.mid2 = nullptr;
+  fp.cap_mid = fp.cap_mid2 = 0;
 }
 
 // launch kern<LOGN> for the runtime log2 of the transformed axis
@@ -410,9 +504,73 @@ FFT2_LAUNCHER(k_fft2_rows_t1)
 FFT2_LAUNCHER(k_fft2_cols_t1)
 FFT2_LAUNCHER(k_fft2_cols_t2)
 FFT2_LAUNCHER(k_fft2_rows_t2)
+FFT2_LAUNCHER(k_fft3_pass)
 #undef FFT2_LAUNCHER
 
+// 3D: three passes (launch one per axis; logn of the transformed axis)
+hipError_t fft3_run(Fft2Plan& fp, const DeconvArgs& a, const void* src, void* dst, int isign, bool to_grid,
+                    hipStream_t s) {
+  const int64_t n0 = a.nm[0], n1 = a.nm[1], n2 = a.nm[2], f0 = a.nf[0], f1 = a.nf[1], f2 = a.nf[2];
+  const int conj = isign > 0 ? 1 : 0;
+  const unsigned nt = (unsigned)a.n_trans;
+  const int64_t mid_t = fft3_mid_cells(a), mid2_t = fft3_mid2_cells(a);
+  const int64_t grid_t = f0 * f1 * f2, mode_t = n0 * n1 * n2;
+  const float* fac[3] = {static_cast<const float*>(a.fac[0]), static_cast<const float*>(a.fac[1]),
+                         static_cast<const float*>(a.fac[2])};
+  auto pass = [&](int axis, Fft3Pass P, const float2* in, float2* out) {
+    const int logn = log2_exact(a.nf[axis]);
+    P.modeord = a.modeord;
+    P.nmode = a.nm[axis];
+    P.tw = static_cast<const float2*>(fp.tw[axis]);
+    const unsigned blocks = (unsigned)((P.nseq + fft_seqs(logn) - 1) / fft_seqs(logn));
+    launch_logn<k_fft3_pass_l>(logn, dim3(blocks, nt), fft_lds(a.nf[axis]), s, P, in, out, conj);
+  };
+  Fft3Pass P{};
+  if (to_grid) {
+    // modes [n2][n1][n0] -> x: [n2][nf0][n1] -> y: [nf1][nf0][n2] -> z: grid [nf2][nf1][nf0]
+    P = Fft3Pass{};
+    P.nin = (int)n0; P.nout = (int)f0; P.in_modes = 1; P.out_modes = 0;
+    P.nseq = n1 * n2; P.D1 = n1; P.P1 = n1 * n0; P.P2 = n0; P.Q1 = f0 * n1; P.Q2 = 1; P.ostride = n1;
+    P.in_t = mode_t; P.out_t = mid_t; P.facA = fac[0]; P.facB = fac[1]; P.facC = fac[2];
+    pass(0, P, static_cast<const float2*>(src), fp.mid);
+    P = Fft3Pass{};
+    P.nin = (int)n1; P.nout = (int)f1; P.in_modes = 1; P.out_modes = 0;
+    P.nseq = f0 * n2; P.D1 = n2; P.P1 = n1; P.P2 = f0 * n1; P.Q1 = n2; P.Q2 = 1; P.ostride = f0 * n2;
+    P.in_t = mid_t; P.out_t = mid2_t;
+    pass(1, P, fp.mid, fp.mid2);
+    P = Fft3Pass{};
+    P.nin = (int)n2; P.nout = (int)f2; P.in_modes = 1; P.out_modes = 0;
+    P.nseq = f0 * f1; P.D1 = f0; P.P1 = f0 * n2; P.P2 = n2; P.Q1 = f0; P.Q2 = 1; P.ostride = f0 * f1;
+    P.in_t = mid2_t; P.out_t = grid_t;
+    pass(2, P, fp.mid2, static_cast<float2*>(dst));
+  } else {
+    // grid [nf2][nf1][nf0] -> x: [nf2][n0][nf1] -> y: [n1][n0][nf2] -> z: modes [n2][n1][n0]
+    P = Fft3Pass{};
+    P.nin = (int)f0; P.nout = (int)n0; P.in_modes = 0; P.out_modes = 1;
+    P.nseq = f1 * f2; P.D1 = f1; P.P1 = f1 * f0; P.P2 = f0; P.Q1 = n0 * f1; P.Q2 = 1; P.ostride = f1;
+    P.in_t = grid_t; P.out_t = mid_t;
+    pass(0, P, static_cast<const float2*>(src), fp.mid);
+    P = Fft3Pass{};
+    P.nin = (int)f1; P.nout = (int)n1; P.in_modes = 0; P.out_modes = 1;
+    P.nseq = n0 * f2; P.D1 = f2; P.P1 = f1; P.P2 = n0 * f1; P.Q1 = f2; P.Q2 = 1; P.ostride = n0 * f2;
+    P.in_t = mid_t; P.out_t = mid2_t;
+    pass(1, P, fp.mid, fp.mid2);
+    P = Fft3Pass{};
+    P.nin = (int)f2; P.nout = (int)n2; P.in_modes = 0; P.out_modes = 1;
+    P.nseq = n0 * n1; P.D1 = n0; P.P1 = n0 * f2; P.P2 = f2; P.Q1 = n0; P.Q2 = 1; P.ostride = n0 * n1;
+    P.in_t = mid2_t; P.out_t = mode_t; P.facA = fac[2]; P.facB = fac[0]; P.facC = fac[1];
+    pass(2, P, fp.mid2, static_cast<float2*>(dst));
+  }
+  return hipGetLastError();
+}
+
 hipError_t fft2_shuffle(Fft2Plan& fp, const DeconvArgs& a, const void* grid, void* f, int isign, hipStream_t s) {
+  if (a.dim == 3) {
+    hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * fft3_mid_cells(a), fp.retired);
+    if (e == hipSuccess) e = grow_buffer(fp.mid2, fp.cap_mid2, a.n_trans * fft3_mid2_cells(a), fp.retired);
+    if (e != hipSuccess) return e;
+    return fft3_run(fp, a, grid, f, isign, false, s);
+  }
   const Fft2K k = make_k(fp, a);
   hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * k.mid_cells, fp.retired);
   if (e != hipSuccess) return e;
@@ -426,6 +584,12 @@ hipError_t fft2_shuffle(Fft2Plan& fp, const DeconvArgs& a, const void* grid, voi
 }
 
 hipError_t fft2_pad(Fft2Plan& fp, const DeconvArgs& a, const void* f, void* grid, int isign, hipStream_t s) {
+  if (a.dim == 3) {
+    hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * fft3_mid_cells(a), fp.retired);
+    if (e == hipSuccess) e = grow_buffer(fp.mid2, fp.cap_mid2, a.n_trans * fft3_mid2_cells(a), fp.retired);
+    if (e != hipSuccess) return e;
+    return fft3_run(fp, a, f, grid, isign, true, s);
+  }
   const Fft2K k = make_k(fp, a);
   hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * k.mid_cells, fp.retired);
   if (e != hipSuccess) return e;

@@ -235,11 +235,12 @@ struct DeconvArgs {
 hipError_t launch_deconv_shuffle(const DeconvArgs& a, const void* fw_hat, void* f, hipStream_t s);
 hipError_t launch_deconv_pad(const DeconvArgs& a, const void* f, void* fw_hat, hipStream_t s);
 
-// ---- fft2d.hip: fused FFT + deconvolution of 2D FP32 plans on power-of-two grids
+// ---- fft2d.hip: fused FFT + deconvolution of 2D and 3D FP32 plans on power-of-two grids
 struct Fft2Plan {
-  void* tw[2] = {nullptr, nullptr};  // float2 twiddle tables, nf_d / 2 entries
-  float2* mid = nullptr;             // n_trans * nm0 * nf1 intermediate (x-modes, transposed)
-  int64_t cap_mid = 0;
+  void* tw[3] = {nullptr, nullptr, nullptr};  // float2 twiddle tables, nf_d / 2 entries
+  float2* mid = nullptr;   // 2D: n_trans * nm0 * nf1 (x-modes, transposed); 3D: first intermediate
+  float2* mid2 = nullptr;  // 3D: second intermediate
+  int64_t cap_mid = 0, cap_mid2 = 0;
   std::vector<void*> retired;
 };
 bool fft2_applicable(const DeconvArgs& a);
