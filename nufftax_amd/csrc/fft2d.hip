@@ -45,6 +45,15 @@ __host__ __device__ constexpr int fft_seq_threads(int logn) {
 }
 __host__ __device__ constexpr int fft_seqs(int logn) { return kFftThreads / fft_seq_threads(logn); }
 
+// 3D passes: as many threads per sequence as the Stockham pass with the fewest columns has
+// columns (every thread busy in every pass; 16 for 256 points, against 32 above), 8 .. 128
+__host__ __device__ constexpr int fft3_seq_threads(int logn) {
+  return ((1 << logn) >> (logn / ((logn + 3) / 4))) < 8
+             ? 8
+             : (((1 << logn) >> (logn / ((logn + 3) / 4))) > 128 ? 128 : ((1 << logn) >> (logn / ((logn + 3) / 4))));
+}
+__host__ __device__ constexpr int fft3_seqs(int logn) { return kFftThreads / fft3_seq_threads(logn); }
+
 __device__ __forceinline__ int fft_mode_to_bin(int m, int n, int nf, int modeord) {
   const int kmin = -(n / 2), kmax = (n - 1) / 2;
   const int k = modeord == 0 ? m + kmin : (m <= kmax ? m : m - n);
@@ -71,12 +80,14 @@ __host__ __device__ constexpr int bitrev_c(int i) {
 }
 
 // LDS layout of a sequence of N = 2^LOGN points: one pad slot after every 2^ceil(LOGN / 2)
-// points, which spreads the strided writes of the first Stockham passes over the banks
-template <int LOGN>
+// points, which spreads the strided writes of the first Stockham passes over the banks; EXTRA
+// more slots at the end of each sequence (1 makes the stride odd, so that reads across
+// sequences at one index fall in different banks)
+template <int LOGN, int EXTRA = 0>
 struct FftLayout {
   static constexpr int L1 = (LOGN + 1) / 2;
   static constexpr int N = 1 << LOGN;
-  static constexpr int PL = N + (N >> L1);  // padded sequence length
+  static constexpr int PL = N + (N >> L1) + EXTRA;  // padded sequence length
   static __device__ __forceinline__ int at(int r, int n) { return r * PL + n + (n >> L1); }
 };
 
@@ -104,12 +115,10 @@ __device__ __forceinline__ void reg_dft(float2 (&v)[1 << L], const float2* tw, i
 
 // One Stockham pass of radix R = 2^L over the S sequences of a block (in place in the padded
 // layout: every read of the pass precedes every write); Ns = 2^LNS.  Ends with a barrier.
-template <int LOGN, int L, int LNS>
+template <int LOGN, int L, int LNS, int T = fft_seq_threads(LOGN), class Lay = FftLayout<LOGN>>
 __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, bool conj) {
   constexpr int N = 1 << LOGN, R = 1 << L, NR = N / R, Ns = 1 << LNS;
-  constexpr int T = fft_seq_threads(LOGN);
   constexpr int J = (NR + T - 1) / T;  // columns per thread
-  using Lay = FftLayout<LOGN>;
   const int seq = threadIdx.x / T, q = threadIdx.x % T;
   float2 v[J][R];
 #pragma unroll
@@ -139,17 +148,17 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, boo
 
 // In-place FFT of the S sequences of N = 2^LOGN points (6 <= LOGN <= 12) of a block, natural
 // order in and out: ceil(LOGN / 4) passes of 3 or 4 bits (the first ones the larger).  Ends
-// with a barrier.
-template <int LOGN>
+// with a barrier.  T threads per sequence, layout Lay.
+template <int LOGN, int T = fft_seq_threads(LOGN), class Lay = FftLayout<LOGN>>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, bool conj) {
   constexpr int NP = (LOGN + 3) / 4;
   constexpr int B0 = LOGN / NP + (0 < LOGN % NP ? 1 : 0);
   constexpr int B1 = LOGN / NP + (1 < LOGN % NP ? 1 : 0);
   constexpr int B2 = LOGN - B0 - B1;
   static_assert(NP >= 2 && NP <= 3 && B0 <= 4 && B1 <= 4 && B2 <= 4, "2 or 3 passes of radix <= 16");
-  stockham_pass<LOGN, B0, 0>(buf, tw, conj);
-  stockham_pass<LOGN, B1, B0>(buf, tw, conj);
-  if constexpr (B2 > 0) stockham_pass<LOGN, B2, B0 + B1>(buf, tw, conj);
+  stockham_pass<LOGN, B0, 0, T, Lay>(buf, tw, conj);
+  stockham_pass<LOGN, B1, B0, T, Lay>(buf, tw, conj);
+  if constexpr (B2 > 0) stockham_pass<LOGN, B2, B0 + B1, T, Lay>(buf, tw, conj);
 }
 
 __device__ __forceinline__ void load_twiddles(float2* stw, const float2* __restrict__ tw, int n) {
@@ -324,8 +333,9 @@ template <int LOGN>
 __global__ void __launch_bounds__(kFftThreads) k_fft3_pass(Fft3Pass P, const float2* __restrict__ in,
                                                            float2* __restrict__ out, int conj) {
   extern __shared__ __align__(16) float2 smem[];
-  constexpr int N = 1 << LOGN, S = fft_seqs(LOGN);
-  using Lay = FftLayout<LOGN>;
+  constexpr int N = 1 << LOGN, T = fft3_seq_threads(LOGN), S = kFftThreads / T;
+  constexpr int EPT = S * N / kFftThreads;  // values per thread in and out
+  using Lay = FftLayout<LOGN, 1>;
   __shared__ int64_t sa[S], sb[S];
   float2* stw = smem;
   float2* buf = smem + (N >> 1);
@@ -342,35 +352,61 @@ __global__ void __launch_bounds__(kFftThreads) k_fft3_pass(Fft3Pass P, const flo
   if (P.in_modes)
     for (int e = threadIdx.x; e < S * Lay::PL; e += kFftThreads) buf[e] = make_float2(0.f, 0.f);
   __syncthreads();
-  for (int e = threadIdx.x; e < S * P.nin; e += kFftThreads) {
-    const int r = e / P.nin, i = e - r * P.nin;
-    if (s0 + r >= P.nseq) continue;
-    float2 v = inT[sa[r] * P.P1 + sb[r] * P.P2 + i];
-    if (P.in_modes) {
-      float f = P.facA ? P.facA[i] : 1.0f;
-      if (P.facB) f *= P.facB[sb[r]];
-      if (P.facC) f *= P.facC[sa[r]];
-      buf[Lay::at(r, fft_mode_to_bin(i, P.nmode, N, P.modeord))] = make_float2(v.x * f, v.y * f);
-    } else {
-      buf[Lay::at(r, i)] = v;
+  // every load of the thread is issued before the first LDS write
+  float2 v[EPT];
+  if (P.in_modes) {
+    const int nin = P.nin;
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = threadIdx.x + u * kFftThreads;
+      const int r = e / nin, i = e - r * nin;
+      v[u] = make_float2(0.f, 0.f);
+      if (r < S && s0 + r < P.nseq) v[u] = inT[sa[r] * P.P1 + sb[r] * P.P2 + i];
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = threadIdx.x + u * kFftThreads;
+      const int r = e / nin, i = e - r * nin;
+      if (r < S && s0 + r < P.nseq) {
+        float f = P.facA ? P.facA[i] : 1.0f;
+        if (P.facB) f *= P.facB[sb[r]];
+        if (P.facC) f *= P.facC[sa[r]];
+        buf[Lay::at(r, fft_mode_to_bin(i, P.nmode, N, P.modeord))] = make_float2(v[u].x * f, v[u].y * f);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = threadIdx.x + u * kFftThreads;
+      const int r = e >> LOGN, i = e & (N - 1);
+      v[u] = make_float2(0.f, 0.f);
+      if (s0 + r < P.nseq) v[u] = inT[sa[r] * P.P1 + sb[r] * P.P2 + i];
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = threadIdx.x + u * kFftThreads;
+      buf[Lay::at(e >> LOGN, e & (N - 1))] = v[u];
     }
   }
   __syncthreads();
-  lds_fft<LOGN>(buf, stw, conj != 0);
-  for (int e = threadIdx.x; e < S * P.nout; e += kFftThreads) {
-    const int j = e / S, r = e - j * S;
-    if (s0 + r >= P.nseq) continue;
-    float2 v;
+  lds_fft<LOGN, T, Lay>(buf, stw, conj != 0);
+  // consecutive lanes take consecutive sequences at one output index: runs of S values
+#pragma unroll
+  for (int u = 0; u < EPT; ++u) {
+    const int e = threadIdx.x + u * kFftThreads;
+    const int j = e / S, r = e & (S - 1);
+    if (j >= P.nout || s0 + r >= P.nseq) continue;
+    float2 w;
     if (P.out_modes) {
-      v = buf[Lay::at(r, fft_mode_to_bin(j, P.nmode, N, P.modeord))];
+      w = buf[Lay::at(r, fft_mode_to_bin(j, P.nmode, N, P.modeord))];
       float f = P.facA ? P.facA[j] : 1.0f;
       if (P.facB) f *= P.facB[sb[r]];
       if (P.facC) f *= P.facC[sa[r]];
-      v = make_float2(v.x * f, v.y * f);
+      w = make_float2(w.x * f, w.y * f);
     } else {
-      v = buf[Lay::at(r, j)];
+      w = buf[Lay::at(r, j)];
     }
-    outT[sa[r] * P.Q1 + sb[r] * P.Q2 + (int64_t)j * P.ostride] = v;
+    outT[sa[r] * P.Q1 + sb[r] * P.Q2 + (int64_t)j * P.ostride] = w;
   }
 }
 
@@ -401,6 +437,13 @@ size_t fft_lds(int n) {
   const int l = log2_exact(n);
   const int pl = n + (n >> ((l + 1) / 2));
   return (size_t)(n / 2 + fft_seqs(l) * pl) * sizeof(float2);
+}
+
+// LDS of a 3D pass over length-n sequences (twiddles + S sequences of odd stride)
+size_t fft3_lds(int n) {
+  const int l = log2_exact(n);
+  const int pl = n + (n >> ((l + 1) / 2)) + 1;
+  return (size_t)(n / 2 + fft3_seqs(l) * pl) * sizeof(float2);
 }
 
 unsigned fft_blocks(int count, int n) {
@@ -522,8 +565,8 @@ hipError_t fft3_run(Fft2Plan& fp, const DeconvArgs& a, const void* src, void* ds
     P.modeord = a.modeord;
     P.nmode = a.nm[axis];
     P.tw = static_cast<const float2*>(fp.tw[axis]);
-    const unsigned blocks = (unsigned)((P.nseq + fft_seqs(logn) - 1) / fft_seqs(logn));
-    launch_logn<k_fft3_pass_l>(logn, dim3(blocks, nt), fft_lds(a.nf[axis]), s, P, in, out, conj);
+    const unsigned blocks = (unsigned)((P.nseq + fft3_seqs(logn) - 1) / fft3_seqs(logn));
+    launch_logn<k_fft3_pass_l>(logn, dim3(blocks, nt), fft3_lds(a.nf[axis]), s, P, in, out, conj);
   };
   Fft3Pass P{};
   if (to_grid) {
