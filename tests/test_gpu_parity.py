@@ -685,14 +685,15 @@ def test_interp2d_f32_wide_reads(rng, nfs, eps, wide, monkeypatch):
 # ------------------------------------------------------------------ fused FFT + deconvolution
 
 
-@pytest.mark.parametrize("nm", [(32, 32, 64), (31, 63, 64)])
+@pytest.mark.parametrize("nm", [(32, 32, 64), (31, 63, 64), (128, 32, 256), (127, 255, 32)])
 @pytest.mark.parametrize("isign", [1, -1])
 @pytest.mark.parametrize("modeord", [0, 1])
 def test_fused_fft3_matches_rocfft(rng, monkeypatch, nm, isign, modeord):
     """3D FP32 plans on power-of-two grids run FFT + deconvolution as three fused Stockham
     passes (default: along x, y, z, each on the kept modes of the axes done before); the same
     plans built with NUFFTX_ROCFFT take rocFFT + the separate deconvolution kernels.  Type 1 and
-    type 2, two transforms per call, odd and even mode counts; relative l2 < 2e-6."""
+    type 2, two transforms per call, odd and even mode counts, axes of 64 .. 512 cells (each
+    thread layout of the passes: 8, 16 and 64 threads per sequence); relative l2 < 2e-6."""
     from nufftax_amd._plan import F32, Plan
 
     M = 20000
