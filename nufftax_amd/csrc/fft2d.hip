@@ -1,6 +1,7 @@
 // Fused uniform-grid stage of 2D and 3D FP32 type-1 and type-2 transforms on power-of-two fine
-// grids (64 .. 4096 per axis; 3D: k_fft3_pass, three passes, described there): the c2c FFT of the fine grid (nufftax/transforms/nufft1.py:81,
-// nufft2.py:74: fftn for isign < 0, unnormalised ifftn * N for isign > 0) together with the
+// grids (64 .. 4096 per axis; 3D: k_fft3_pass, three passes, described there): the c2c FFT of
+// the fine grid (nufftax/transforms/nufft1.py:81, nufft2.py:74: fftn for isign < 0,
+// unnormalised ifftn * N for isign > 0) together with the
 // deconvolution, mode truncation and shift of deconvolve.py:174-224 (type 1) or the
 // deconvolution and zero padding of deconvolve.py:227-304 (type 2).
 //
@@ -23,7 +24,8 @@
 // bit-reversed loads.  Twiddles exp(-2 pi i k / N), k < N/2, come from a table computed on the
 // host in FP64 (staged in LDS) and are conjugated for isign > 0.  A block of kFftThreads
 // threads transforms S sequences with T = min(N / 8, 128) threads each (one 8-point column of
-// a radix-8 pass per thread).  An earlier four-step version (two passes of 32-point register
+// a radix-8 pass per thread; the 3D passes take as many threads as the pass with the fewest
+// columns has columns, 16 for N = 256).  An earlier four-step version (two passes of 32-point register
 // DFTs, 32 threads per 1024-point sequence) left most of the GPU idle: 128 blocks of two waves
 // for the 512 kept columns of a 1024^2 grid.
 #include <algorithm>
