@@ -362,6 +362,63 @@ def test_wrap_invariance_and_empty(rng):
     assert e.shape == (0,)
 
 
+@pytest.mark.parametrize("dim,nm", [(1, (64,)), (2, (64, 64)), (3, (32, 32, 32))])
+def test_special_strengths_and_coordinates(rng, dim, nm):
+    """tests/test_edge_cases.py:123-197, :386-425 through each dimension's default complex64
+    path (2D, 3D: the matrix-core window spreads, whose fixed-point scale follows the largest
+    strength): all-zero strengths give exact zeros; one nonzero strength equals the one-point
+    transform (rtol 1e-5); strengths and modes scaled by 1e-15 and 1e10 (the reference's) and by
+    1e-30 and 1e30 give finite results equal to the scaled ones (relative l2 < 1e-6); points at
+    +-(pi - 1e-6) match the FP64 direct sum."""
+    import nufftax_amd as N
+
+    fn1 = {1: N.nufft1d1, 2: N.nufft2d1, 3: N.nufft3d1}[dim]
+    fn2 = {1: N.nufft1d2, 2: N.nufft2d2, 3: N.nufft3d2}[dim]
+    n_modes = nm[0] if dim == 1 else nm
+    M = 100
+    x = [rng.uniform(-np.pi, np.pi, M).astype(np.float32) for _ in range(dim)]
+    xs = [_t(v) for v in x]
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    shape = tuple(reversed(nm))
+    f = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+    zero = fn1(*xs, _t(np.zeros(M, np.complex64)), n_modes, eps=1e-6)
+    assert float(zero.abs().max()) == 0.0
+    one = np.zeros(M, np.complex64)
+    one[0] = 1 + 2j
+    a = _np(fn1(*xs, _t(one), n_modes, eps=1e-6))
+    b = _np(fn1(*[v[:1] for v in xs], _t(one[:1]), n_modes, eps=1e-6))
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-5 * np.abs(b).max())
+
+    base1 = _np(fn1(*xs, _t(c), n_modes, eps=1e-6))
+    base2 = _np(fn2(*xs, _t(f), eps=1e-6))
+    for sc in (1e-15, 1e10, 1e-30, 1e30):
+        o1 = _np(fn1(*xs, _t((c * sc).astype(np.complex64)), n_modes, eps=1e-6)).astype(np.complex128)
+        o2 = _np(fn2(*xs, _t((f * sc).astype(np.complex64)), eps=1e-6)).astype(np.complex128)
+        assert np.all(np.isfinite(o1)) and np.all(np.isfinite(o2)), sc
+        assert R.rel_l2(o1 / sc, base1) < 1e-6, sc
+        assert R.rel_l2(o2 / sc, base2) < 1e-6, sc
+
+    edge = np.concatenate([np.full(M // 2, np.pi - 1e-6), np.full(M - M // 2, -np.pi + 1e-6)]).astype(np.float32)
+    xe = [edge] + [rng.uniform(-np.pi, np.pi, M).astype(np.float32) for _ in range(dim - 1)]
+    o = _np(fn1(*[_t(v) for v in xe], _t(c), n_modes, eps=1e-6))
+    assert np.all(np.isfinite(o))
+    ref = R.direct_type1([v.astype(np.float64) for v in xe], c.astype(np.complex128), nm, isign=1)
+    assert R.rel_l2(o, ref) < 1e-5
+
+
+def test_3d_single_point_closed_form():
+    """tests/test_edge_cases.py:362-380 in complex64: f[k3, k2, k1] = c exp(i (k1 x + k2 y + k3 z))."""
+    import nufftax_amd as N
+
+    v = lambda a: _t(np.array([a], np.float32))  # noqa: E731
+    out = _np(N.nufft3d1(v(0.5), v(0.3), v(0.1), _t(np.array([1 + 1j], np.complex64)), (4, 4, 4), eps=1e-6))
+    k = np.arange(-2, 2)
+    expected = (1 + 1j) * np.exp(1j * (k[None, None, :] * 0.5 + k[None, :, None] * 0.3 + k[:, None, None] * 0.1))
+    assert out.shape == (4, 4, 4)
+    np.testing.assert_allclose(out, expected, rtol=1e-5, atol=1e-5)
+
+
 def test_batched_n_trans_matches_loop(rng):
     import nufftax_amd as N
 
