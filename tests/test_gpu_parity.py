@@ -407,6 +407,40 @@ def test_special_strengths_and_coordinates(rng, dim, nm):
     assert R.rel_l2(o, ref) < 1e-5
 
 
+@pytest.mark.parametrize("dim,nm", [(1, (64,)), (2, (64, 64)), (3, (32, 32, 32))])
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+def test_non_finite_inputs(rng, dim, nm, bad):
+    """A non-finite strength poisons a type-1 result as in the reference (spread.py:466-489
+    scatters it into the grid, the FFT carries it into every mode): every mode is NaN.  A
+    non-finite coordinate is handled without a fault and leaves the other points' type-2
+    values equal to the all-finite result; the value of that point itself, and its type-1
+    contribution, are not pinned (the reference's ES weights come out 0 for it,
+    kernel.py:101-111; here it depends on the path)."""
+    import nufftax_amd as N
+
+    fn1 = {1: N.nufft1d1, 2: N.nufft2d1, 3: N.nufft3d1}[dim]
+    fn2 = {1: N.nufft1d2, 2: N.nufft2d2, 3: N.nufft3d2}[dim]
+    n_modes = nm[0] if dim == 1 else nm
+    M = 2000
+    x = [rng.uniform(-np.pi, np.pi, M).astype(np.float32) for _ in range(dim)]
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    shape = tuple(reversed(nm))
+    f = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    cb = c.copy()
+    cb[11] = complex(bad, 0.0)
+    assert np.all(np.isnan(_np(fn1(*[_t(v) for v in x], _t(cb), n_modes, eps=1e-6))))
+
+    xb = [v.copy() for v in x]
+    xb[dim - 1][7] = bad
+    o1 = fn1(*[_t(v) for v in xb], _t(c), n_modes, eps=1e-6)
+    assert o1.shape == shape
+    good = _np(fn2(*[_t(v) for v in x], _t(f), eps=1e-6))
+    got = _np(fn2(*[_t(v) for v in xb], _t(f), eps=1e-6))
+    keep = np.arange(M) != 7
+    assert np.all(np.isfinite(got[keep]))
+    assert R.rel_l2(got[keep], good[keep]) < 1e-6
+
+
 def test_3d_single_point_closed_form():
     """tests/test_edge_cases.py:362-380 in complex64: f[k3, k2, k1] = c exp(i (k1 x + k2 y + k3 z))."""
     import nufftax_amd as N
