@@ -31,6 +31,7 @@ from typing import NamedTuple, Sequence
 
 import numpy as np
 import torch
+from torch.autograd import forward_ad as _fwad
 from torch.autograd.function import once_differentiable
 
 from . import _lib
@@ -467,6 +468,30 @@ class _Type3(torch.autograd.Function):
         return _unbatched_apply(lambda *a: _Type3.apply(*a), info, in_dims, (meta, c) + pts)
 
 
+# ---------------------------------------------------------------- dispatch
+
+_functorch_active = getattr(torch._C, "_are_functorch_transforms_active", None)
+
+
+def _autodiff_active(tensors) -> bool:
+    """Whether a call must go through its autograd Function: an input that requires grad under
+    grad mode, an open forward-mode dual level, or an active torch.func transform (vmap, jvp,
+    grad).  Function.apply costs ~25 us of host time per call, more than the GPU work of a small
+    transform, so calls with none of these run the forward directly (the same GPU path).  When
+    the state cannot be read, the Function is used."""
+    if _functorch_active is None or _functorch_active():
+        return True
+    if getattr(_fwad, "_current_level", 0) >= 0:
+        return True
+    return torch.is_grad_enabled() and any(t.requires_grad for t in tensors)
+
+
+def _run(fn, meta, src, *ts):
+    if _autodiff_active((src,) + ts):
+        return fn.apply(meta, src, *ts)
+    return fn.forward(meta, src, *ts)
+
+
 # ---------------------------------------------------------------- input normalisation
 
 
@@ -500,7 +525,7 @@ def nufft1d1(x, c, n_modes: int, eps: float = 1e-6, isign: int = 1, upsampfac=2.
     (x, c), restore = _prepare([x, c])
     uf, uf_t = _split_upsampfac(upsampfac)
     meta = _Meta(1, _modes_tuple(n_modes, 1), float(eps), 1 if isign > 0 else -1, uf, uf_t)
-    return restore(_Type1.apply(meta, _as_complex(c), x))
+    return restore(_run(_Type1, meta, _as_complex(c), x))
 
 
 def nufft2d1(x, y, c, n_modes, eps: float = 1e-6, isign: int = 1, upsampfac=2.0):
@@ -508,7 +533,7 @@ def nufft2d1(x, y, c, n_modes, eps: float = 1e-6, isign: int = 1, upsampfac=2.0)
     (x, y, c), restore = _prepare([x, y, c])
     uf, uf_t = _split_upsampfac(upsampfac)
     meta = _Meta(2, _modes_tuple(n_modes, 2), float(eps), 1 if isign > 0 else -1, uf, uf_t)
-    return restore(_Type1.apply(meta, _as_complex(c), x, y))
+    return restore(_run(_Type1, meta, _as_complex(c), x, y))
 
 
 def nufft3d1(x, y, z, c, n_modes, eps: float = 1e-6, isign: int = 1, upsampfac=2.0):
@@ -516,7 +541,7 @@ def nufft3d1(x, y, z, c, n_modes, eps: float = 1e-6, isign: int = 1, upsampfac=2
     (x, y, z, c), restore = _prepare([x, y, z, c])
     uf, uf_t = _split_upsampfac(upsampfac)
     meta = _Meta(3, _modes_tuple(n_modes, 3), float(eps), 1 if isign > 0 else -1, uf, uf_t)
-    return restore(_Type1.apply(meta, _as_complex(c), x, y, z))
+    return restore(_run(_Type1, meta, _as_complex(c), x, y, z))
 
 
 def nufft1d2(x, f, eps: float = 1e-6, isign: int = -1, upsampfac=2.0):
@@ -524,7 +549,7 @@ def nufft1d2(x, f, eps: float = 1e-6, isign: int = -1, upsampfac=2.0):
     (x, f), restore = _prepare([x, f])
     uf, uf_t = _split_upsampfac(upsampfac)
     meta = _Meta(1, (f.shape[-1],), float(eps), 1 if isign > 0 else -1, uf, uf_t)
-    return restore(_Type2.apply(meta, _as_complex(f), x))
+    return restore(_run(_Type2, meta, _as_complex(f), x))
 
 
 def nufft2d2(x, y, f, eps: float = 1e-6, isign: int = -1, upsampfac=2.0):
@@ -532,7 +557,7 @@ def nufft2d2(x, y, f, eps: float = 1e-6, isign: int = -1, upsampfac=2.0):
     (x, y, f), restore = _prepare([x, y, f])
     uf, uf_t = _split_upsampfac(upsampfac)
     meta = _Meta(2, (f.shape[-1], f.shape[-2]), float(eps), 1 if isign > 0 else -1, uf, uf_t)
-    return restore(_Type2.apply(meta, _as_complex(f), x, y))
+    return restore(_run(_Type2, meta, _as_complex(f), x, y))
 
 
 def nufft3d2(x, y, z, f, eps: float = 1e-6, isign: int = -1, upsampfac=2.0):
@@ -540,26 +565,26 @@ def nufft3d2(x, y, z, f, eps: float = 1e-6, isign: int = -1, upsampfac=2.0):
     (x, y, z, f), restore = _prepare([x, y, z, f])
     uf, uf_t = _split_upsampfac(upsampfac)
     meta = _Meta(3, (f.shape[-1], f.shape[-2], f.shape[-3]), float(eps), 1 if isign > 0 else -1, uf, uf_t)
-    return restore(_Type2.apply(meta, _as_complex(f), x, y, z))
+    return restore(_run(_Type2, meta, _as_complex(f), x, y, z))
 
 
 def nufft1d3(x, c, s, n_modes: int, eps: float = 1e-6, isign: int = 1, upsampfac: float = 2.0):
     """f[k] = sum_j c[j] exp(i isign s[k] x[j]); n_modes = fine-grid size (compute_type3_grid_size)."""
     (x, c, s), restore = _prepare([x, c, s])
     meta = _Meta(1, (int(n_modes),), float(eps), 1 if isign > 0 else -1, float(upsampfac), float(upsampfac))
-    return restore(_Type3.apply(meta, _as_complex(c), x, s))
+    return restore(_run(_Type3, meta, _as_complex(c), x, s))
 
 
 def nufft2d3(x, y, c, s, t, n_modes, eps: float = 1e-6, isign: int = 1, upsampfac: float = 2.0):
     (x, y, c, s, t), restore = _prepare([x, y, c, s, t])
     meta = _Meta(2, _modes_tuple(n_modes, 2), float(eps), 1 if isign > 0 else -1, float(upsampfac), float(upsampfac))
-    return restore(_Type3.apply(meta, _as_complex(c), x, y, s, t))
+    return restore(_run(_Type3, meta, _as_complex(c), x, y, s, t))
 
 
 def nufft3d3(x, y, z, c, s, t, u, n_modes, eps: float = 1e-6, isign: int = 1, upsampfac: float = 2.0):
     (x, y, z, c, s, t, u), restore = _prepare([x, y, z, c, s, t, u])
     meta = _Meta(3, _modes_tuple(n_modes, 3), float(eps), 1 if isign > 0 else -1, float(upsampfac), float(upsampfac))
-    return restore(_Type3.apply(meta, _as_complex(c), x, y, z, s, t, u))
+    return restore(_run(_Type3, meta, _as_complex(c), x, y, z, s, t, u))
 
 
 # Backward-compatibility aliases (autodiff.py:151-161)

@@ -319,3 +319,35 @@ def test_serving_admission_skips_requests_that_do_not_fit():
     units, pending, nb = srv._next_group(pending)
     t.join()
     assert released and units == [[big]] and pending == [s3] and srv._in_use == 800
+
+
+def test_autodiff_dispatch_predicate():
+    """The public transforms skip their autograd Function only when nothing can differentiate
+    through the call: no input requiring grad under grad mode, no forward-mode dual level, no
+    torch.func transform (vmap, jvp, grad)."""
+    import torch
+    from torch.autograd import forward_ad as fwad
+
+    from nufftax_amd.transforms import _autodiff_active
+
+    x = torch.zeros(4)
+    assert not _autodiff_active([x])
+    w = torch.zeros(4, requires_grad=True)
+    assert _autodiff_active([x, w])
+    with torch.no_grad():
+        assert not _autodiff_active([x, w])
+    with torch.inference_mode():
+        assert not _autodiff_active([x])
+    with fwad.dual_level():
+        assert _autodiff_active([x])
+    assert not _autodiff_active([x])
+    seen = []
+
+    def probe(v):
+        seen.append(_autodiff_active([v]))
+        return v * 2
+
+    torch.func.vmap(probe)(torch.zeros(3, 4))
+    torch.func.jvp(probe, (x,), (torch.ones(4),))
+    torch.func.grad(lambda v: probe(v).sum())(x)
+    assert seen == [True, True, True]
