@@ -1,5 +1,6 @@
-// Fused uniform-grid stage of 2D and 3D FP32 type-1 and type-2 transforms on power-of-two fine
-// grids (64 .. 4096 per axis; 3D: k_fft3_pass, three passes, described there): the c2c FFT of
+// Fused uniform-grid stage of 1D, 2D and 3D FP32 type-1 and type-2 transforms on power-of-two
+// fine grids (64 .. 4096 per axis; 3D: k_fft3_pass, three passes, described there; 1D: one
+// k_fft3_pass per transform): the c2c FFT of
 // the fine grid (nufftax/transforms/nufft1.py:81, nufft2.py:74: fftn for isign < 0,
 // unnormalised ifftn * N for isign > 0) together with the
 // deconvolution, mode truncation and shift of deconvolve.py:174-224 (type 1) or the
@@ -460,10 +461,18 @@ void fft_lds_attr(const void* kern, size_t b) {
 }  // namespace
 
 bool fft2_applicable(const DeconvArgs& a) {
-  if ((a.dim != 2 && a.dim != 3) || a.prec != kF32 || a.from_phihat) return false;
+  if (a.dim < 1 || a.dim > 3 || a.prec != kF32 || a.from_phihat) return false;
   if (a.dim == 3)
     if (const char* e = std::getenv("NUFFTX_FFT3_FUSED"))
       if (e[0] == '0') return false;
+  if (a.dim == 1) {
+    // one block per transform with a single busy sequence: it removes a launch and rocFFT's
+    // host-side cost per call (1D 100k points, 1024 modes: 57 -> 50 us through the public
+    // transform, +3 us of GPU time); large batches of short transforms stay with rocFFT
+    if (a.n_trans > 64) return false;
+    if (const char* e = std::getenv("NUFFTX_FFT1_FUSED"))
+      if (e[0] == '0') return false;
+  }
   for (int d = 0; d < a.dim; ++d) {
     const int l = log2_exact(a.nf[d]);
     if (l < 6 || l > 12 || a.nm[d] > a.nf[d]) return false;
@@ -487,7 +496,8 @@ hipError_t fft2_plan_init(Fft2Plan& fp, const DeconvArgs& a) {
   if (a.dim == 3) {
     if ((e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * fft3_mid_cells(a), fp.retired)) != hipSuccess) return e;
     if ((e = grow_buffer(fp.mid2, fp.cap_mid2, a.n_trans * fft3_mid2_cells(a), fp.retired)) != hipSuccess) return e;
-  } else if ((e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * (int64_t)a.nm[0] * a.nf[1], fp.retired)) != hipSuccess) {
+  } else if (a.dim == 2 &&
+             (e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * (int64_t)a.nm[0] * a.nf[1], fp.retired)) != hipSuccess) {
     return e;
   }
   for (int d = 0; d < a.dim; ++d) {
@@ -609,7 +619,33 @@ hipError_t fft3_run(Fft2Plan& fp, const DeconvArgs& a, const void* src, void* ds
   return hipGetLastError();
 }
 
+// 1D: one pass per transform (one sequence; the k_fft3_pass block's other slots stay idle):
+// grid -> kept modes times fac (type 1), or modes times fac at their bins -> grid (type 2)
+hipError_t fft1_run(Fft2Plan& fp, const DeconvArgs& a, const void* src, void* dst, int isign, bool to_grid,
+                    hipStream_t s) {
+  const int64_t n = a.nm[0], nf = a.nf[0];
+  const int logn = log2_exact(a.nf[0]);
+  Fft3Pass P{};
+  P.nin = (int)(to_grid ? n : nf);
+  P.nout = (int)(to_grid ? nf : n);
+  P.in_modes = to_grid ? 1 : 0;
+  P.out_modes = to_grid ? 0 : 1;
+  P.nmode = (int)n;
+  P.modeord = a.modeord;
+  P.nseq = 1;
+  P.D1 = 1;
+  P.ostride = 1;
+  P.in_t = to_grid ? n : nf;
+  P.out_t = to_grid ? nf : n;
+  P.facA = static_cast<const float*>(a.fac[0]);
+  P.tw = static_cast<const float2*>(fp.tw[0]);
+  launch_logn<k_fft3_pass_l>(logn, dim3(1, (unsigned)a.n_trans), fft3_lds(a.nf[0]), s, P,
+                             static_cast<const float2*>(src), static_cast<float2*>(dst), isign > 0 ? 1 : 0);
+  return hipGetLastError();
+}
+
 hipError_t fft2_shuffle(Fft2Plan& fp, const DeconvArgs& a, const void* grid, void* f, int isign, hipStream_t s) {
+  if (a.dim == 1) return fft1_run(fp, a, grid, f, isign, false, s);
   if (a.dim == 3) {
     hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * fft3_mid_cells(a), fp.retired);
     if (e == hipSuccess) e = grow_buffer(fp.mid2, fp.cap_mid2, a.n_trans * fft3_mid2_cells(a), fp.retired);
@@ -629,6 +665,7 @@ hipError_t fft2_shuffle(Fft2Plan& fp, const DeconvArgs& a, const void* grid, voi
 }
 
 hipError_t fft2_pad(Fft2Plan& fp, const DeconvArgs& a, const void* f, void* grid, int isign, hipStream_t s) {
+  if (a.dim == 1) return fft1_run(fp, a, f, grid, isign, true, s);
   if (a.dim == 3) {
     hipError_t e = grow_buffer(fp.mid, fp.cap_mid, a.n_trans * fft3_mid_cells(a), fp.retired);
     if (e == hipSuccess) e = grow_buffer(fp.mid2, fp.cap_mid2, a.n_trans * fft3_mid2_cells(a), fp.retired);

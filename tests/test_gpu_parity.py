@@ -810,6 +810,46 @@ def _check_fused_vs_rocfft(fused, rocfft, refs, sigma):
 
 
 @pytest.mark.parametrize(
+    "nm,sigma", [((32,), 2.0), ((63,), 2.0), ((1024,), 2.0), ((2047,), 2.0), ((51,), 1.25), ((409,), 1.25)]
+)
+@pytest.mark.parametrize("isign", [1, -1])
+@pytest.mark.parametrize("modeord", [0, 1])
+def test_fused_fft1_matches_rocfft(rng, monkeypatch, nm, sigma, isign, modeord):
+    """1D FP32 plans on power-of-two grids (64 .. 4096 cells here) run FFT + deconvolution as one
+    fused Stockham pass per transform (default); the same plans built with NUFFTX_ROCFFT take
+    rocFFT + the separate deconvolution kernels.  Type 1 and type 2, two transforms per call,
+    odd and even mode counts, upsampfac 2 and 1.25; bars as in the 2D and 3D tests."""
+    from nufftax_amd._plan import F32, Plan
+
+    M = 20000
+    xs = [_t(rng.uniform(-np.pi, np.pi, M).astype(np.float32))]
+    c = _t((rng.standard_normal((2, M)) + 1j * rng.standard_normal((2, M))).astype(np.complex64))
+    f = _t((rng.standard_normal((2, nm[0])) + 1j * rng.standard_normal((2, nm[0]))).astype(np.complex64))
+
+    def run(rocfft):
+        if rocfft:
+            monkeypatch.setenv("NUFFTX_ROCFFT", "1")
+        else:
+            monkeypatch.delenv("NUFFTX_ROCFFT", raising=False)
+        p1 = Plan.transform(1, 1, nm, 2, 1e-6, isign, sigma, F32, modeord, torch.device(DEV))
+        p2 = Plan.transform(2, 1, nm, 2, 1e-6, -isign, sigma, F32, modeord, torch.device(DEV))
+        assert p1.grid_size(0) == 2 ** int(np.log2(p1.grid_size(0)))
+        p1.setpts(xs)
+        p2.setpts(xs)
+        o1 = torch.empty((2, nm[0]), dtype=torch.complex64, device=DEV)
+        o2 = torch.empty((2, M), dtype=torch.complex64, device=DEV)
+        p1.execute(c, o1)
+        p2.execute(o2, f)
+        torch.cuda.synchronize()
+        refs = None if sigma == 2.0 else _fft_stage_refs(p1, p2, c, f, nm, isign, sigma, modeord)
+        return (_np(o1), _np(o2)), refs
+
+    fused, refs = run(False)
+    rocfft, _ = run(True)
+    _check_fused_vs_rocfft(fused, rocfft, refs or (None, None), sigma)
+
+
+@pytest.mark.parametrize(
     "nm,sigma",
     [((32, 32, 64), 2.0), ((31, 63, 64), 2.0), ((128, 32, 256), 2.0), ((127, 255, 32), 2.0),
      ((51, 102, 204), 1.25)],
