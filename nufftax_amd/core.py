@@ -43,6 +43,7 @@ from . import _lib
 from ._lib import F32, F64, check, lib, ptr
 from ._plan import _stream_handle, grid_plan
 from .params import KernelParams, compute_kernel_params
+from .transforms import _run
 
 __all__ = [
     "es_kernel", "es_kernel_with_derivative", "compute_kernel_params", "kernel_fourier_series", "KernelParams",
@@ -313,7 +314,7 @@ def _spread(dim, coords, c, nf, kernel_params):
     batched = c.ndim == 2
     c2 = c if batched else c[None, :]
     gctx = _GridCtx(dim, tuple(int(v) for v in nf), _es_params(kernel_params))
-    fw = _Spread.apply(gctx, c2, *coords)
+    fw = _run(_Spread, gctx, c2, *coords)
     if not batched:
         fw = fw[0]
     if was_numpy:
@@ -333,7 +334,7 @@ def _interp(dim, coords, fw, kernel_params):
     fw2 = fw if batched else fw[None]
     nf = tuple(reversed(fw2.shape[1:]))
     gctx = _GridCtx(dim, nf, _es_params(kernel_params))
-    c = _Interp.apply(gctx, fw2, *coords)
+    c = _run(_Interp, gctx, fw2, *coords)
     if not batched:
         c = c[0]
     if was_numpy:
